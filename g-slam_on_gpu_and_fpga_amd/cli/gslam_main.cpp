@@ -1,0 +1,209 @@
+// GSLAM -- drop-in command-line driver over libgslam.
+//
+//   GSLAM <use_diff> <lambda> <num_rounds> <drawing_mode> <input_file>
+//
+// Same argv, stdout lines and pose dump as the reference executables
+// (src/cpu_pcg_fp64/GSLAM.c:13-220, src/gpu_pcg_fp64/GSLAM.cu:28-335):
+//   "Diff: %s Lambda: %s Rounds: %s Drawing: %s Input file: %s"
+//   "delta: %f size_of_us: %d size_of_hat_xs: %d size_of_zlist: %d dim: %d"
+//   "%d rounds executed: %.17g"                       (one per GN round)
+// and, when drawing_mode >= 1 and POSE_DUMP_FILE is set, the final poses as
+// "%u %.17g %.17g %.17g" lines.  Stopping: diff < 0.01, or 20 rounds without
+// a new best diff (GSLAM.c:119-130).  As in the reference GPU driver
+// (src/gpu_pcg_fp64/GSLAM.cu:165-171) the diff is always ||dx||/sqrt(dim).
+//
+// Extra knobs (environment, defaults reproduce the reference):
+//   GSLAM_SOLVER   pcg64 | mixed | gj64        (pcg64)
+//   GSLAM_TOL      PCG relative tolerance      (1e-6, MatrixOps.c:345)
+//   GSLAM_MAX_ITER PCG iteration cap           (500,  MatrixOps.c:347)
+//   GSLAM_WRAP     1: wrap theta residuals and poses (0)
+//   GSLAM_ORDER    landmark pair order 0 (CPU reference) | 1 (GPU reference)
+//   GSLAM_STATS    1: one JSON line per round on stderr
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gslam.h"
+
+namespace {
+
+struct LogData {
+    double delta = 0.0;
+    std::vector<unsigned int> u_step, x_step, z_step, z_lm;
+    std::vector<double> u_nu, u_omega, poses, z_range, z_bearing;
+};
+
+// read_data, src/cpu_pcg_fp64/GSLAM.c:415-493: ' '-separated tokens, atoi /
+// atof conversions, any first token other than u/x/z sets delta from the
+// second token.  A line too short for its record is an error here (the
+// reference dereferences a NULL token).
+int read_log(const char *path, LogData &d)
+{
+    FILE *f = std::fopen(path, "r");
+    if (!f) {
+        std::printf("Error! Cannot open the input file!");
+        return -1;
+    }
+    char line[1024];
+    int lineno = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++lineno;
+        char *tok = std::strtok(line, " ");
+        const int need = !tok ? 0 : (!std::strcmp(tok, "u") ? 3 : (!std::strcmp(tok, "x") || !std::strcmp(tok, "z")) ? 4 : 1);
+        char *t[4] = {nullptr, nullptr, nullptr, nullptr};
+        bool ok = tok != nullptr;
+        for (int i = 0; ok && i < need; ++i) ok = (t[i] = std::strtok(nullptr, " ")) != nullptr;
+        if (!ok) {
+            std::fprintf(stderr, "GSLAM: malformed record at %s:%d\n", path, lineno);
+            std::fclose(f);
+            return -2;
+        }
+        if (!std::strcmp(tok, "u")) {
+            d.u_step.push_back(static_cast<unsigned int>(std::atoi(t[0])));
+            d.u_nu.push_back(std::atof(t[1]));
+            d.u_omega.push_back(std::atof(t[2]));
+        } else if (!std::strcmp(tok, "x")) {
+            d.x_step.push_back(static_cast<unsigned int>(std::atoi(t[0])));
+            d.poses.push_back(std::atof(t[1]));
+            d.poses.push_back(std::atof(t[2]));
+            d.poses.push_back(std::atof(t[3]));
+        } else if (!std::strcmp(tok, "z")) {
+            d.z_step.push_back(static_cast<unsigned int>(std::atoi(t[0])));
+            d.z_lm.push_back(static_cast<unsigned int>(std::atoi(t[1])));
+            d.z_range.push_back(std::atof(t[2]));
+            d.z_bearing.push_back(std::atof(t[3]));
+        } else {
+            d.delta = std::atof(t[0]);
+        }
+    }
+    std::fclose(f);
+    return 0;
+}
+
+double env_double(const char *name, double dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atof(v) : dflt;
+}
+
+int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[])
+{
+    if (argc == 6) {
+        std::printf("Diff: %s ", argv[1]);
+        std::printf("Lambda: %s ", argv[2]);
+        std::printf("Rounds: %s ", argv[3]);
+        std::printf("Drawing: %s ", argv[4]);
+        std::printf("Input file: %s\n", argv[5]);
+    } else {
+        std::printf("Usage: ");
+        std::printf("%s Diff_0|1 Lambda Number_of_rounds Drawing_0|1|2|3 path/to/input/file\n", argv[0]);
+        return 1;
+    }
+    const double lambda = std::atof(argv[2]);
+    const unsigned int rounds = static_cast<unsigned int>(std::atoi(argv[3]));
+    const unsigned int draw = static_cast<unsigned int>(std::atoi(argv[4]));
+
+    LogData d;
+    if (read_log(argv[5], d) != 0) return 1;
+    const unsigned int dim = static_cast<unsigned int>(d.u_nu.size()) * 3;
+    std::printf("delta: %f ", d.delta);
+    std::printf("size_of_us: %d ", static_cast<int>(d.u_nu.size()));
+    std::printf("size_of_hat_xs: %d ", static_cast<int>(d.x_step.size()));
+    std::printf("size_of_zlist: %d ", static_cast<int>(d.z_step.size()));
+    std::printf("dim: %d\n", dim);
+    std::fflush(stdout);
+
+    const int n = static_cast<int>(d.x_step.size());
+    if (n == 0 || d.u_nu.size() != d.x_step.size()) {
+        std::fprintf(stderr, "GSLAM: need one u record per x record (u=%zu, x=%zu)\n",
+                     d.u_nu.size(), d.x_step.size());
+        return 1;
+    }
+    std::vector<int32_t> zs(d.z_step.begin(), d.z_step.end()), zl(d.z_lm.begin(), d.z_lm.end());
+
+    gslam_graph_desc desc;
+    gslam_default_desc(&desc);
+    desc.n_poses = n;
+    desc.poses = d.poses.data();
+    desc.motion = 1;
+    desc.u_nu = d.u_nu.data();
+    desc.u_omega = d.u_omega.data();
+    desc.delta = d.delta;
+    desc.lambda = lambda;
+    desc.n_obs = static_cast<int32_t>(zs.size());
+    desc.z_step = zs.data();
+    desc.z_lm = zl.data();
+    desc.z_range = d.z_range.data();
+    desc.z_bearing = d.z_bearing.data();
+    desc.pair_order = env_int("GSLAM_ORDER", 0);
+    desc.wrap = env_int("GSLAM_WRAP", 0);
+
+    gslam_graph *g = nullptr;
+    int rc = gslam_create(&desc, &g);
+    if (rc != GSLAM_OK) {
+        std::fprintf(stderr, "GSLAM: gslam_create failed (%d): %s\n", rc, gslam_last_error());
+        return 1;
+    }
+    gslam_solve_opts opts;
+    gslam_default_opts(&opts);
+    const char *solver = std::getenv("GSLAM_SOLVER");
+    if (solver && !std::strcmp(solver, "mixed")) opts.solver = GSLAM_SOLVER_PCG_MIXED;
+    if (solver && !std::strcmp(solver, "gj64")) opts.solver = GSLAM_SOLVER_GJ_FP64;
+    opts.tol = env_double("GSLAM_TOL", opts.tol);
+    opts.max_iter = env_int("GSLAM_MAX_ITER", opts.max_iter);
+    const bool stats_json = env_int("GSLAM_STATS", 0) != 0;
+
+    double best = 1e30;
+    unsigned int no_improve = 0;
+    for (unsigned int r = 1; r <= rounds; ++r) {
+        gslam_solve_stats st;
+        double diff = 0.0;
+        rc = gslam_gn_step(g, &opts, &st, &diff);
+        if (rc != GSLAM_OK && rc != GSLAM_ERR_NOT_CONVERGED && rc != GSLAM_ERR_SINGULAR) {
+            std::fprintf(stderr, "GSLAM: GN step failed (%d): %s\n", rc, gslam_last_error());
+            gslam_destroy(g);
+            return 1;
+        }
+        std::printf("%d rounds executed: %.17g\n", r, diff);
+        std::fflush(stdout);
+        if (stats_json)
+            std::fprintf(stderr,
+                         "{\"round\": %u, \"iters\": %d, \"converged\": %d, \"rel_res\": %.6e, "
+                         "\"true_rel_res\": %.6e, \"solve_ms\": %.4f}\n",
+                         r, st.iters, st.converged, st.rel_res, st.true_rel_res, st.ms);
+        if (diff < 0.01) break;
+        if (diff < best) {
+            best = diff;
+            no_improve = 0;
+        } else if (++no_improve >= 20) {
+            break;
+        }
+    }
+    if (draw >= 1) {
+        const char *dump = std::getenv("POSE_DUMP_FILE");
+        if (dump) {
+            std::vector<double> poses(3 * static_cast<size_t>(n));
+            gslam_get_poses(g, poses.data());
+            FILE *df = std::fopen(dump, "w");
+            if (df) {
+                for (int i = 0; i < n; ++i)
+                    std::fprintf(df, "%u %.17g %.17g %.17g\n", d.x_step[i], poses[3 * i],
+                                 poses[3 * i + 1], poses[3 * i + 2]);
+                std::fclose(df);
+            }
+        }
+    }
+    gslam_destroy(g);
+    return 0;
+}

@@ -1,0 +1,416 @@
+// C-ABI of libgslam (include/gslam.h): handle lifecycle, uploads, and the
+// GN-step entry points over the assembly / PCG / GJ kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gslam.h"
+#include "gslam_internal.h"
+#include "pcg_common.h"
+
+namespace gslam {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+DeviceGraph::~DeviceGraph()
+{
+    for (void *p : allocs) (void)hipFree(p);
+    if (status_host) (void)hipHostFree(status_host);
+    if (status_ring) (void)hipHostFree(status_ring);
+    if (ev_poll[0]) (void)hipEventDestroy(ev_poll[0]);
+    if (ev_poll[1]) (void)hipEventDestroy(ev_poll[1]);
+    if (scal_host) (void)hipHostFree(scal_host);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int DeviceGraph::upload(void *dst, const void *src, int64_t nbytes)
+{
+    if (nbytes <= 0) return GSLAM_OK;
+    GSLAM_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(nbytes), hipMemcpyHostToDevice, stream));
+    GSLAM_HIP(hipStreamSynchronize(stream));
+    return GSLAM_OK;
+}
+
+int DeviceGraph::download(void *dst, const void *src, int64_t nbytes)
+{
+    if (nbytes <= 0) return GSLAM_OK;
+    GSLAM_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(nbytes), hipMemcpyDeviceToHost, stream));
+    GSLAM_HIP(hipStreamSynchronize(stream));
+    return GSLAM_OK;
+}
+
+static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
+{
+    std::string err;
+    int rc = build_host_graph(d, g.h, err);
+    if (rc != GSLAM_OK) {
+        set_error(err);
+        return rc;
+    }
+    const HostGraph &h = g.h;
+    g.lambda = d.lambda;
+    g.anchor = d.anchor;
+    g.delta = d.delta;
+    for (int k = 0; k < 4; ++k) g.mns[k] = d.mns[k];
+    g.snr[0] = d.snr[0];
+    g.snr[1] = d.snr[1];
+    g.wrap = d.wrap ? 1 : 0;
+
+    GSLAM_HIP(hipGetDevice(&g.dev));
+    GSLAM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    GSLAM_HIP(hipEventCreate(&g.ev0));
+    GSLAM_HIP(hipEventCreate(&g.ev1));
+    GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[0], hipEventDisableTiming));
+    GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[1], hipEventDisableTiming));
+    GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.status_host), sizeof(PcgStatus)));
+    GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.status_ring), 2 * sizeof(PcgStatus)));
+    GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.scal_host), 16 * sizeof(double)));
+
+    const int64_t N = h.n, dim = 3 * N, E = h.n_e, B = h.n_blocks;
+    GSLAM_TRY(g.alloc(&g.poses, dim));
+    GSLAM_TRY(g.alloc(&g.dx, dim));
+    GSLAM_TRY(g.alloc(&g.e_t1, E));
+    GSLAM_TRY(g.alloc(&g.e_t2, E));
+    GSLAM_TRY(g.alloc(&g.obs_param, 4 * h.n_obs_e));
+    GSLAM_TRY(g.alloc(&g.mot_param, 2 * h.n_mot_e));
+    GSLAM_TRY(g.alloc(&g.bet_param, 6 * h.n_bet_e));
+    GSLAM_TRY(g.alloc(&g.bet_w, h.n_bet_e));
+    GSLAM_TRY(g.alloc(&g.brow_ptr, N + 1));
+    GSLAM_TRY(g.alloc(&g.bcol, B));
+    GSLAM_TRY(g.alloc(&g.bdiag, N));
+    GSLAM_TRY(g.alloc(&g.cptr, B + 1));
+    GSLAM_TRY(g.alloc(&g.clist, 4 * E));
+    GSLAM_TRY(g.alloc(&g.xptr, N + 1));
+    GSLAM_TRY(g.alloc(&g.xlist, 2 * E));
+    GSLAM_TRY(g.alloc(&g.csr_row_ptr, dim + 1));
+    GSLAM_TRY(g.alloc(&g.csr_col_idx, h.nnz));
+    GSLAM_TRY(g.alloc(&g.csr_pos, 9 * B));
+    GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
+    GSLAM_TRY(g.alloc(&g.bval, 9 * B));
+    GSLAM_TRY(g.alloc(&g.bval32, 9 * B));
+    GSLAM_TRY(g.alloc(&g.xi, dim));
+    GSLAM_TRY(g.alloc(&g.x, dim));
+    GSLAM_TRY(g.alloc(&g.r, dim));
+    GSLAM_TRY(g.alloc(&g.z, dim));
+    GSLAM_TRY(g.alloc(&g.q, dim));
+    GSLAM_TRY(g.alloc(&g.p[0], dim));
+    GSLAM_TRY(g.alloc(&g.p[1], dim));
+    GSLAM_TRY(g.alloc(&g.dinv, 9 * N));
+    GSLAM_TRY(g.alloc(&g.dinv32, 9 * N));
+    GSLAM_TRY(g.alloc(&g.r32, dim));
+    GSLAM_TRY(g.alloc(&g.z32, dim));
+    GSLAM_TRY(g.alloc(&g.q32, dim));
+    GSLAM_TRY(g.alloc(&g.p32[0], dim));
+    GSLAM_TRY(g.alloc(&g.p32[1], dim));
+    GSLAM_TRY(g.alloc(&g.r64, dim));
+    GSLAM_TRY(g.alloc(&g.xoth, 2 * E));
+    GSLAM_TRY(g.alloc(&g.xpos, 2 * E));
+    GSLAM_TRY(g.alloc(&g.frec, 16 * E));
+    GSLAM_TRY(g.alloc(&g.part, kPartTotal));
+    GSLAM_TRY(g.alloc(&g.status, 1));
+    GSLAM_TRY(g.alloc(&g.scal, 16));
+
+    GSLAM_TRY(g.upload(g.poses, d.poses, dim * 8));
+    GSLAM_TRY(g.upload(g.e_t1, h.t1.data(), E * 4));
+    GSLAM_TRY(g.upload(g.e_t2, h.t2.data(), E * 4));
+    GSLAM_TRY(g.upload(g.obs_param, h.obs_param.data(), h.obs_param.size() * 8));
+    GSLAM_TRY(g.upload(g.mot_param, h.mot_param.data(), h.mot_param.size() * 8));
+    GSLAM_TRY(g.upload(g.bet_param, h.bet_param.data(), h.bet_param.size() * 8));
+    {
+        std::vector<double> ones(static_cast<size_t>(h.n_bet_e), 1.0);
+        GSLAM_TRY(g.upload(g.bet_w, ones.data(), h.n_bet_e * 8));
+    }
+    GSLAM_TRY(g.upload(g.brow_ptr, h.brow_ptr.data(), (N + 1) * 4));
+    GSLAM_TRY(g.upload(g.bcol, h.bcol.data(), B * 4));
+    GSLAM_TRY(g.upload(g.bdiag, h.bdiag.data(), N * 4));
+    GSLAM_TRY(g.upload(g.cptr, h.cptr.data(), (B + 1) * 4));
+    GSLAM_TRY(g.upload(g.clist, h.clist.data(), 4 * E * 4));
+    GSLAM_TRY(g.upload(g.xptr, h.xptr.data(), (N + 1) * 4));
+    GSLAM_TRY(g.upload(g.xlist, h.xlist.data(), 2 * E * 4));
+    GSLAM_TRY(g.upload(g.csr_row_ptr, h.csr_row_ptr.data(), (dim + 1) * 4));
+    GSLAM_TRY(g.upload(g.csr_col_idx, h.csr_col_idx.data(), h.nnz * 4));
+    GSLAM_TRY(g.upload(g.csr_pos, h.csr_pos.data(), 9 * B * 4));
+    GSLAM_TRY(g.upload(g.xoth, h.xoth.data(), 2 * E * 4));
+    GSLAM_TRY(g.upload(g.xpos, h.xpos.data(), 2 * E * 4));
+    return GSLAM_OK;
+}
+
+}  // namespace gslam
+
+using gslam::DeviceGraph;
+using gslam::set_error;
+
+extern "C" {
+
+void gslam_default_opts(gslam_solve_opts *o)
+{
+    if (!o) return;
+    o->solver = GSLAM_SOLVER_PCG_FP64;
+    o->tol = 1e-6;
+    o->max_iter = 500;
+    o->refine_sweeps = 3;
+    o->refresh_every = 64;
+    o->check_every = 0;
+}
+
+void gslam_default_desc(gslam_graph_desc *d)
+{
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->delta = 1.0;
+    d->lambda = 0.1;
+    d->mns[0] = 0.19;
+    d->mns[1] = 0.001;
+    d->mns[2] = 0.13;
+    d->mns[3] = 0.2;
+    d->snr[0] = 0.14;
+    d->snr[1] = 0.05;
+    d->anchor = 1e6;
+    d->pair_order = 0;
+}
+
+const char *gslam_last_error(void) { return gslam::g_last_error.c_str(); }
+
+int gslam_create(const gslam_graph_desc *desc, gslam_graph **out)
+{
+    if (!desc || !out) {
+        set_error("gslam_create: NULL argument");
+        return GSLAM_ERR_ARG;
+    }
+    *out = nullptr;
+    gslam_graph *h = new (std::nothrow) gslam_graph();
+    if (!h) return GSLAM_ERR_NOMEM;
+    int rc = gslam::create_device(*desc, h->g);
+    if (rc != GSLAM_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return GSLAM_OK;
+}
+
+int gslam_destroy(gslam_graph *g)
+{
+    delete g;
+    return GSLAM_OK;
+}
+
+int gslam_get_info(const gslam_graph *gh, gslam_graph_info *info)
+{
+    if (!gh || !info) return GSLAM_ERR_ARG;
+    const auto &h = gh->g.h;
+    info->n_poses = h.n;
+    info->n_edges = h.n_e;
+    info->n_obs_edges = h.n_obs_e;
+    info->n_motion_edges = h.n_mot_e;
+    info->n_between_edges = h.n_bet_e;
+    info->n_blocks = h.n_blocks;
+    info->nnz = h.nnz;
+    info->device_bytes = gh->g.bytes;
+    return GSLAM_OK;
+}
+
+int gslam_set_poses(gslam_graph *gh, const double *poses)
+{
+    if (!gh || !poses) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    g.assembled = g.solved = false;
+    return g.upload(g.poses, poses, 3 * static_cast<int64_t>(g.h.n) * 8);
+}
+
+int gslam_get_poses(gslam_graph *gh, double *poses)
+{
+    if (!gh || !poses) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    return g.download(poses, g.poses, 3 * static_cast<int64_t>(g.h.n) * 8);
+}
+
+int gslam_set_lambda(gslam_graph *gh, double lambda)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    gh->g.lambda = lambda;
+    gh->g.assembled = gh->g.solved = false;
+    return GSLAM_OK;
+}
+
+int gslam_set_log_weights(gslam_graph *gh, const double *log_w)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    std::vector<double> w(static_cast<size_t>(g.h.n_bet_e), 1.0);
+    if (log_w)
+        for (size_t k = 0; k < w.size(); ++k) w[k] = std::exp(log_w[k]);
+    g.assembled = g.solved = false;
+    return g.upload(g.bet_w, w.data(), g.h.n_bet_e * 8);
+}
+
+int gslam_assemble(gslam_graph *gh)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    GSLAM_TRY(gslam::launch_assembly(g));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    g.assembled = true;
+    g.solved = false;
+    return GSLAM_OK;
+}
+
+int gslam_get_system_csr(gslam_graph *gh, int32_t *row_ptr, int32_t *col_idx, double *values,
+                         double *xi)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    const auto &h = g.h;
+    if ((values || xi) && !g.assembled) {
+        set_error("gslam_get_system_csr: call gslam_assemble first");
+        return GSLAM_ERR_STATE;
+    }
+    if (row_ptr) std::memcpy(row_ptr, h.csr_row_ptr.data(), h.csr_row_ptr.size() * 4);
+    if (col_idx) std::memcpy(col_idx, h.csr_col_idx.data(), h.csr_col_idx.size() * 4);
+    if (values) {
+        double *tmp = nullptr;
+        GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&tmp),
+                            static_cast<size_t>(h.nnz > 0 ? h.nnz : 1) * 8));
+        int rc = gslam::launch_bsr_to_csr(g, g.bval, tmp);
+        if (rc == GSLAM_OK) rc = g.download(values, tmp, h.nnz * 8);
+        (void)hipFree(tmp);
+        if (rc != GSLAM_OK) return rc;
+    }
+    if (xi) GSLAM_TRY(g.download(xi, g.xi, 3 * static_cast<int64_t>(h.n) * 8));
+    return GSLAM_OK;
+}
+
+int gslam_get_system_bsr(gslam_graph *gh, int32_t *brow_ptr, int32_t *bcol, double *bval,
+                         double *xi)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    const auto &h = g.h;
+    if ((bval || xi) && !g.assembled) {
+        set_error("gslam_get_system_bsr: call gslam_assemble first");
+        return GSLAM_ERR_STATE;
+    }
+    if (brow_ptr) std::memcpy(brow_ptr, h.brow_ptr.data(), h.brow_ptr.size() * 4);
+    if (bcol) std::memcpy(bcol, h.bcol.data(), h.bcol.size() * 4);
+    if (bval) GSLAM_TRY(g.download(bval, g.bval, 9 * h.n_blocks * 8));
+    if (xi) GSLAM_TRY(g.download(xi, g.xi, 3 * static_cast<int64_t>(h.n) * 8));
+    return GSLAM_OK;
+}
+
+static int check_opts(const gslam_solve_opts *o)
+{
+    if (!o) return GSLAM_ERR_ARG;
+    if (o->solver < GSLAM_SOLVER_PCG_FP64 || o->solver > GSLAM_SOLVER_GJ_MIXED) {
+        set_error("unknown solver id");
+        return GSLAM_ERR_ARG;
+    }
+    if (!(o->tol >= 0.0) || o->max_iter < 0) {
+        set_error("tol must be >= 0 and max_iter >= 0");
+        return GSLAM_ERR_ARG;
+    }
+    return GSLAM_OK;
+}
+
+static int solve_into(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                      gslam_solve_stats &s)
+{
+    std::memset(&s, 0, sizeof(s));
+    if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED)
+        return gslam::dense_solve(g, o, b, x, s);
+    GSLAM_TRY(gslam::pcg_solve(g, o, b, x, s));
+    if (!s.converged && !(s.flags & 1) && o.max_iter > 0 && s.iters >= o.max_iter)
+        return GSLAM_ERR_NOT_CONVERGED;
+    return GSLAM_OK;
+}
+
+int gslam_solve(gslam_graph *gh, const gslam_solve_opts *opts, gslam_solve_stats *stats)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    GSLAM_TRY(check_opts(opts));
+    DeviceGraph &g = gh->g;
+    if (!g.assembled) {
+        set_error("gslam_solve: call gslam_assemble first");
+        return GSLAM_ERR_STATE;
+    }
+    gslam_solve_stats s;
+    int rc = solve_into(g, *opts, g.xi, g.dx, s);
+    if (stats) *stats = s;
+    if (rc == GSLAM_OK || rc == GSLAM_ERR_NOT_CONVERGED) g.solved = true;
+    return rc;
+}
+
+int gslam_solve_rhs(gslam_graph *gh, const gslam_solve_opts *opts, const double *rhs, double *x,
+                    gslam_solve_stats *stats)
+{
+    if (!gh || !rhs || !x) return GSLAM_ERR_ARG;
+    GSLAM_TRY(check_opts(opts));
+    DeviceGraph &g = gh->g;
+    if (!g.assembled) {
+        set_error("gslam_solve_rhs: call gslam_assemble first");
+        return GSLAM_ERR_STATE;
+    }
+    const int64_t dim = 3 * static_cast<int64_t>(g.h.n);
+    double *b = nullptr, *xd = nullptr;
+    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&b), static_cast<size_t>(dim) * 8));
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&xd), static_cast<size_t>(dim) * 8);
+    if (e != hipSuccess) {
+        (void)hipFree(b);
+        set_error(hipGetErrorString(e));
+        return GSLAM_ERR_NOMEM;
+    }
+    gslam_solve_stats s;
+    int rc = g.upload(b, rhs, dim * 8);
+    if (rc == GSLAM_OK) rc = solve_into(g, *opts, b, xd, s);
+    if (rc == GSLAM_OK || rc == GSLAM_ERR_NOT_CONVERGED) {
+        int rc2 = g.download(x, xd, dim * 8);
+        if (rc2 != GSLAM_OK) rc = rc2;
+    }
+    (void)hipFree(b);
+    (void)hipFree(xd);
+    if (stats) *stats = s;
+    return rc;
+}
+
+int gslam_get_dx(gslam_graph *gh, double *dx)
+{
+    if (!gh || !dx) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    if (!g.solved) {
+        set_error("gslam_get_dx: no solution yet");
+        return GSLAM_ERR_STATE;
+    }
+    return g.download(dx, g.dx, 3 * static_cast<int64_t>(g.h.n) * 8);
+}
+
+int gslam_apply_update(gslam_graph *gh, double *diff_out)
+{
+    if (!gh) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    if (!g.solved) {
+        set_error("gslam_apply_update: no solution yet");
+        return GSLAM_ERR_STATE;
+    }
+    GSLAM_TRY(gslam::pose_update(g, diff_out));
+    g.assembled = g.solved = false;
+    return GSLAM_OK;
+}
+
+int gslam_gn_step(gslam_graph *gh, const gslam_solve_opts *opts, gslam_solve_stats *stats,
+                  double *diff_out)
+{
+    GSLAM_TRY(gslam_assemble(gh));
+    int rc = gslam_solve(gh, opts, stats);
+    if (rc != GSLAM_OK && rc != GSLAM_ERR_NOT_CONVERGED) return rc;
+    int rc2 = gslam_apply_update(gh, diff_out);
+    return rc2 != GSLAM_OK ? rc2 : rc;
+}
+
+}  // extern "C"

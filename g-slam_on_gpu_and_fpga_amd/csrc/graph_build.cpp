@@ -1,0 +1,250 @@
+// Host-side structure build for libgslam.  Replaces the reference's
+// dim x dim byte marker (src/cpu_pcg_fp64/MatrixOps.c:177-221,
+// src/gpu_pcg_fp64/device_graph.cu:553-615), which overflows 32-bit indices
+// past N ~ 15k poses, by per-block-row neighbour sorting: O(E log deg)
+// time, O(N + E) memory, 64-bit counts throughout.
+#include "graph_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <unordered_map>
+
+namespace gslam {
+
+void landmark_pairs(int32_t n_obs, const int32_t *z_lm, int order,
+                    std::vector<int32_t> &a, std::vector<int32_t> &b)
+{
+    a.clear();
+    b.clear();
+    if (n_obs <= 0) return;
+    // rank of each landmark's bucket: first appearance or ascending id
+    std::unordered_map<int32_t, int64_t> rank;
+    rank.reserve(static_cast<size_t>(n_obs) * 2);
+    for (int32_t i = 0; i < n_obs; ++i) rank.emplace(z_lm[i], static_cast<int64_t>(i));
+    if (order == 1) {
+        std::vector<int32_t> ids;
+        ids.reserve(rank.size());
+        for (auto &kv : rank) ids.push_back(kv.first);
+        std::sort(ids.begin(), ids.end());
+        for (size_t k = 0; k < ids.size(); ++k) rank[ids[k]] = static_cast<int64_t>(k);
+    }
+    std::vector<int32_t> idx(static_cast<size_t>(n_obs));
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        return rank[z_lm[x]] < rank[z_lm[y]];
+    });
+    size_t lo = 0;
+    while (lo < idx.size()) {
+        size_t hi = lo;
+        while (hi < idx.size() && z_lm[idx[hi]] == z_lm[idx[lo]]) ++hi;
+        for (size_t j = lo; j < hi; ++j)
+            for (size_t k = j + 1; k < hi; ++k) {
+                a.push_back(idx[j]);
+                b.push_back(idx[k]);
+            }
+        lo = hi;
+    }
+}
+
+static int fail(std::string &err, int code, const std::string &msg)
+{
+    err = msg;
+    return code;
+}
+
+int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
+{
+    if (d.n_poses <= 0) return fail(err, GSLAM_ERR_ARG, "n_poses must be > 0");
+    const int32_t N = d.n_poses;
+    g.n = N;
+    if (!d.poses) return fail(err, GSLAM_ERR_ARG, "poses is NULL");
+
+    // ---- observation pairs (ObsEdges) ----
+    std::vector<int32_t> pa, pb;
+    if (d.n_obs > 0) {
+        if (!d.z_step || !d.z_lm || !d.z_range || !d.z_bearing)
+            return fail(err, GSLAM_ERR_ARG, "observation arrays missing");
+        for (int32_t i = 0; i < d.n_obs; ++i)
+            if (d.z_step[i] < 0 || d.z_step[i] >= N)
+                return fail(err, GSLAM_ERR_ARG, "z_step[" + std::to_string(i) +
+                            "] outside [0, n_poses)");
+        landmark_pairs(d.n_obs, d.z_lm, d.pair_order, pa, pb);
+    }
+    g.n_obs_e = static_cast<int64_t>(pa.size());
+    g.n_mot_e = (d.motion && N >= 2) ? N - 1 : 0;
+    if (g.n_mot_e && (!d.u_nu || !d.u_omega))
+        return fail(err, GSLAM_ERR_ARG, "motion edges need u_nu/u_omega");
+    if (g.n_mot_e && !(d.delta > 0.0))
+        return fail(err, GSLAM_ERR_ARG, "delta must be > 0 for motion edges");
+    g.n_bet_e = d.n_between > 0 ? d.n_between : 0;
+    if (g.n_bet_e && (!d.be_i || !d.be_j || !d.be_meas || !d.be_info))
+        return fail(err, GSLAM_ERR_ARG, "between-edge arrays missing");
+    g.n_e = g.n_obs_e + g.n_mot_e + g.n_bet_e;
+    // contribution entries are int32 (e*4+q)
+    if (g.n_e > (std::numeric_limits<int32_t>::max() / 4))
+        return fail(err, GSLAM_ERR_TOO_LARGE, "more than 536,870,911 edges");
+
+    g.t1.resize(g.n_e);
+    g.t2.resize(g.n_e);
+    g.obs_param.resize(4 * g.n_obs_e);
+    for (int64_t e = 0; e < g.n_obs_e; ++e) {
+        int32_t a = pa[e], b = pb[e];
+        g.t1[e] = d.z_step[a];
+        g.t2[e] = d.z_step[b];
+        g.obs_param[4 * e + 0] = d.z_range[a];
+        g.obs_param[4 * e + 1] = d.z_bearing[a];
+        g.obs_param[4 * e + 2] = d.z_range[b];
+        g.obs_param[4 * e + 3] = d.z_bearing[b];
+    }
+    g.mot_param.resize(2 * g.n_mot_e);
+    for (int64_t i = 0; i < g.n_mot_e; ++i) {
+        int64_t e = g.n_obs_e + i;
+        g.t1[e] = static_cast<int32_t>(i);
+        g.t2[e] = static_cast<int32_t>(i + 1);
+        g.mot_param[2 * i + 0] = d.u_nu[i + 1];
+        g.mot_param[2 * i + 1] = d.u_omega[i + 1];
+    }
+    g.bet_param.resize(6 * g.n_bet_e);
+    for (int64_t k = 0; k < g.n_bet_e; ++k) {
+        int64_t e = g.n_obs_e + g.n_mot_e + k;
+        int32_t i = d.be_i[k], j = d.be_j[k];
+        if (i < 0 || i >= N || j < 0 || j >= N)
+            return fail(err, GSLAM_ERR_ARG, "between edge " + std::to_string(k) +
+                        " references a pose outside [0, n_poses)");
+        g.t1[e] = i;
+        g.t2[e] = j;
+        for (int c = 0; c < 3; ++c) {
+            g.bet_param[6 * k + c] = d.be_meas[3 * k + c];
+            g.bet_param[6 * k + 3 + c] = d.be_info[3 * k + c];
+        }
+    }
+
+    // ---- BSR pattern: per block row, sorted unique neighbour columns ----
+    std::vector<int64_t> deg(static_cast<size_t>(N) + 1, 0);
+    std::vector<uint8_t> touched(static_cast<size_t>(N), 0);
+    for (int64_t e = 0; e < g.n_e; ++e) {
+        deg[g.t1[e] + 1] += 2;
+        deg[g.t2[e] + 1] += 2;
+        touched[g.t1[e]] = touched[g.t2[e]] = 1;
+    }
+    for (int32_t i = 0; i < N; ++i) deg[i + 1] += 1;  // the diagonal block
+    for (int32_t i = 0; i < N; ++i) deg[i + 1] += deg[i];
+    std::vector<int32_t> nb(static_cast<size_t>(deg[N]));
+    std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+    for (int32_t i = 0; i < N; ++i) nb[fill[i]++] = i;
+    for (int64_t e = 0; e < g.n_e; ++e) {
+        int32_t a = g.t1[e], b = g.t2[e];
+        nb[fill[a]++] = a;
+        nb[fill[a]++] = b;
+        nb[fill[b]++] = a;
+        nb[fill[b]++] = b;
+    }
+    g.brow_ptr.assign(static_cast<size_t>(N) + 1, 0);
+    g.bcol.clear();
+    g.bcol.reserve(static_cast<size_t>(deg[N]));
+    g.bdiag.resize(N);
+    for (int32_t i = 0; i < N; ++i) {
+        auto first = nb.begin() + deg[i], last = nb.begin() + deg[i + 1];
+        std::sort(first, last);
+        last = std::unique(first, last);
+        for (auto it = first; it != last; ++it) {
+            if (*it == i) g.bdiag[i] = static_cast<int32_t>(g.bcol.size());
+            g.bcol.push_back(*it);
+        }
+        if (g.bcol.size() > static_cast<size_t>(std::numeric_limits<int32_t>::max() / 9))
+            return fail(err, GSLAM_ERR_TOO_LARGE, "more than 238M 3x3 blocks");
+        g.brow_ptr[i + 1] = static_cast<int32_t>(g.bcol.size());
+    }
+    g.n_blocks = static_cast<int64_t>(g.bcol.size());
+
+    auto slot = [&](int32_t row, int32_t col) -> int32_t {
+        auto first = g.bcol.begin() + g.brow_ptr[row];
+        auto last = g.bcol.begin() + g.brow_ptr[row + 1];
+        auto it = std::lower_bound(first, last, col);
+        return static_cast<int32_t>(it - g.bcol.begin());
+    };
+
+    // ---- contribution lists in edge order (stable counting sort) ----
+    const int64_t B = g.n_blocks;
+    std::vector<int32_t> eslot(static_cast<size_t>(4 * g.n_e));
+    g.cptr.assign(static_cast<size_t>(B) + 1, 0);
+    for (int64_t e = 0; e < g.n_e; ++e) {
+        int32_t a = g.t1[e], b = g.t2[e];
+        int32_t s[4] = {g.bdiag[a], slot(a, b), slot(b, a), g.bdiag[b]};
+        for (int q = 0; q < 4; ++q) {
+            eslot[4 * e + q] = s[q];
+            g.cptr[s[q] + 1]++;
+        }
+    }
+    for (int64_t b = 0; b < B; ++b) g.cptr[b + 1] += g.cptr[b];
+    g.clist.resize(static_cast<size_t>(4 * g.n_e));
+    {
+        std::vector<int32_t> f(g.cptr.begin(), g.cptr.end() - 1);
+        for (int64_t e = 0; e < g.n_e; ++e)
+            for (int q = 0; q < 4; ++q)
+                g.clist[f[eslot[4 * e + q]]++] = static_cast<int32_t>(4 * e + q);
+    }
+    g.xptr.assign(static_cast<size_t>(N) + 1, 0);
+    for (int64_t e = 0; e < g.n_e; ++e) {
+        g.xptr[g.t1[e] + 1]++;
+        g.xptr[g.t2[e] + 1]++;
+    }
+    for (int32_t i = 0; i < N; ++i) g.xptr[i + 1] += g.xptr[i];
+    g.xlist.resize(static_cast<size_t>(2 * g.n_e));
+    {
+        std::vector<int32_t> f(g.xptr.begin(), g.xptr.end() - 1);
+        g.xoth.resize(static_cast<size_t>(2 * g.n_e));
+        g.xpos.resize(static_cast<size_t>(2 * g.n_e));
+        for (int64_t e = 0; e < g.n_e; ++e) {
+            const int32_t k0 = f[g.t1[e]]++;
+            g.xlist[k0] = static_cast<int32_t>(2 * e);
+            g.xoth[k0] = g.t2[e];
+            g.xpos[2 * e] = k0;
+            const int32_t k1 = f[g.t2[e]]++;
+            g.xlist[k1] = static_cast<int32_t>(2 * e + 1);
+            g.xoth[k1] = g.t1[e];
+            g.xpos[2 * e + 1] = k1;
+        }
+    }
+
+    // ---- reference CSR layout: only blocks some edge touches, plus the
+    // three anchor diagonal entries of pose 0 when no edge touches it.
+    std::vector<uint8_t> btouched(static_cast<size_t>(B), 0);
+    for (int64_t e = 0; e < 4 * g.n_e; ++e) btouched[eslot[e]] = 1;
+    const int64_t dim = 3 * static_cast<int64_t>(N);
+    g.csr_row_ptr.assign(static_cast<size_t>(dim) + 1, 0);
+    g.csr_pos.assign(static_cast<size_t>(9 * B), -1);
+    int64_t pos = 0;
+    g.csr_col_idx.clear();
+    for (int32_t i = 0; i < N; ++i) {
+        int64_t cnt = 0;
+        for (int32_t b = g.brow_ptr[i]; b < g.brow_ptr[i + 1]; ++b) cnt += btouched[b];
+        for (int a = 0; a < 3; ++a) {
+            if (i == 0 && !touched[0]) {
+                // anchor only (MatrixOps.c:180-182): one diagonal entry per row
+                g.csr_pos[9 * g.bdiag[0] + 4 * a] = static_cast<int32_t>(pos);
+                g.csr_col_idx.push_back(a);
+                ++pos;
+            } else {
+                for (int32_t b = g.brow_ptr[i]; b < g.brow_ptr[i + 1]; ++b) {
+                    if (!btouched[b]) continue;
+                    for (int c = 0; c < 3; ++c) {
+                        g.csr_pos[9 * b + 3 * a + c] = static_cast<int32_t>(pos);
+                        g.csr_col_idx.push_back(3 * g.bcol[b] + c);
+                        ++pos;
+                    }
+                }
+            }
+            if (pos > std::numeric_limits<int32_t>::max())
+                return fail(err, GSLAM_ERR_TOO_LARGE, "CSR nnz exceeds 2^31-1");
+            g.csr_row_ptr[3 * i + a + 1] = static_cast<int32_t>(pos);
+        }
+        (void)cnt;
+    }
+    g.nnz = pos;
+    return GSLAM_OK;
+}
+
+}  // namespace gslam

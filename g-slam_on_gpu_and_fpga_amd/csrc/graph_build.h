@@ -1,0 +1,61 @@
+// Host-side graph structure of libgslam: the edge list in summation order,
+// the 3x3 block-sparse (BSR) pattern of Omega, the per-block contribution
+// lists that make assembly a deterministic gather, and the map to the
+// reference's scalar CSR layout.  Pure C++ (no HIP), built once per graph.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gslam.h"
+
+namespace gslam {
+
+// Quadrant ids of an edge's four 3x3 blocks: (t1,t1) (t1,t2) (t2,t1) (t2,t2).
+enum { Q_UL = 0, Q_UR = 1, Q_BL = 2, Q_BR = 3 };
+// Per-edge staged values: 4 blocks x 9 + 2 x 3 xi = 42 doubles, padded to 48
+// (384 bytes = three 128-byte cache lines per edge record).
+constexpr int kStage = 48;
+
+struct HostGraph {
+    int32_t n = 0;                 // poses
+    int64_t n_obs_e = 0, n_mot_e = 0, n_bet_e = 0, n_e = 0;
+    std::vector<int32_t> t1, t2;   // n_e, summation order
+    std::vector<double> obs_param; // 4 per obs edge: r1 phi1 r2 phi2
+    std::vector<double> mot_param; // 2 per motion edge: nu omega (u[i+1])
+    std::vector<double> bet_param; // 6 per between edge: meas[3] info[3]
+
+    // BSR pattern: every pose owns its diagonal block; off-diagonal blocks
+    // for each distinct neighbour; columns sorted within a block row.
+    std::vector<int32_t> brow_ptr; // n+1
+    std::vector<int32_t> bcol;     // n_blocks
+    std::vector<int32_t> bdiag;    // n: index of block (i,i)
+    int64_t n_blocks = 0;
+
+    // Contributions: block b receives stage quadrants clist[cptr[b]..cptr[b+1])
+    // (entries e*4+q in edge order); pose i receives xi halves
+    // xlist[xptr[i]..xptr[i+1]) (entries e*2+side in edge order).
+    std::vector<int32_t> cptr, clist, xptr, xlist;
+    // Per incidence entry k of xlist: the other pose of the edge; and the
+    // inverse map xpos[2e+side] = k.
+    std::vector<int32_t> xoth, xpos;
+
+    // Reference CSR (src/cpu_pcg_fp64/MatrixOps.c:173-221 pattern): row_ptr,
+    // col_idx and, per BSR value slot (9*b+k), its CSR position or -1.
+    std::vector<int32_t> csr_row_ptr, csr_col_idx, csr_pos;
+    int64_t nnz = 0;
+};
+
+// Validates the description and builds everything above.  Returns GSLAM_OK
+// or a negative status with a message in *err.
+int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err);
+
+// Landmark-pair enumeration (observation index pairs j<k per landmark).
+// order 0: landmarks by first appearance (src/cpu_pcg_fp64/GSLAM.c:246-276);
+// order 1: ascending landmark id (src/gpu_pcg_fp64/device_graph.cu:505-551).
+// Observations keep file order inside a landmark in both.
+void landmark_pairs(int32_t n_obs, const int32_t *z_lm, int order,
+                    std::vector<int32_t> &a, std::vector<int32_t> &b);
+
+}  // namespace gslam

@@ -1,0 +1,147 @@
+// Internal state of a libgslam graph handle and the kernel launchers each
+// translation unit provides.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gslam.h"
+#include "graph_build.h"
+
+namespace gslam {
+
+void set_error(const std::string &msg);
+
+#define GSLAM_HIP(expr)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::gslam::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return GSLAM_ERR_HIP;                                              \
+        }                                                                      \
+    } while (0)
+
+#define GSLAM_TRY(expr)                                                        \
+    do {                                                                       \
+        int r_ = (expr);                                                       \
+        if (r_ != GSLAM_OK) return r_;                                         \
+    } while (0)
+
+// Solver status written by workgroup 0 of the PCG kernels and polled by the
+// host (device memory, mirrored into pinned host memory).
+struct PcgStatus {
+    int32_t done;        // 1: stop (converged, cap, or p.q == 0)
+    int32_t converged;
+    int32_t iters;       // iterations completed
+    int32_t flags;       // bit0: p.q == 0 (MatrixOps.c:357)
+    int32_t refreshes;   // mixed: FP64 true-residual refreshes
+    int32_t pad;
+    double rnorm;        // ||r|| of the last test
+    double bnorm;        // ||b||
+    double tol;          // tol_rel * ||b||
+};
+
+struct DeviceGraph {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    HostGraph h;
+    double lambda = 0.1, anchor = 1e6, delta = 1.0, mns[4] = {0.19, 0.001, 0.13, 0.2};
+    double snr[2] = {0.14, 0.05};
+    int wrap = 0;
+    int64_t bytes = 0;
+    bool assembled = false, solved = false;
+
+    // inputs / poses
+    double *poses = nullptr;     // 3N
+    double *dx = nullptr;        // 3N
+    int32_t *e_t1 = nullptr, *e_t2 = nullptr;
+    double *obs_param = nullptr, *mot_param = nullptr, *bet_param = nullptr;
+    double *bet_w = nullptr;     // per between edge exp(w_e)
+
+    // structure
+    int32_t *brow_ptr = nullptr, *bcol = nullptr, *bdiag = nullptr;
+    int32_t *cptr = nullptr, *clist = nullptr, *xptr = nullptr, *xlist = nullptr;
+    int32_t *csr_row_ptr = nullptr, *csr_col_idx = nullptr, *csr_pos = nullptr;
+
+    // assembled system
+    double *stage = nullptr;     // kStage per edge
+    double *bval = nullptr;      // 9 per block, FP64
+    float *bval32 = nullptr;     // 9 per block, FP32 copy (mixed solver)
+    double *xi = nullptr;        // 3N
+
+    // solver work space
+    double *x = nullptr, *r = nullptr, *z = nullptr, *q = nullptr;
+    double *p[2] = {nullptr, nullptr};
+    double *dinv = nullptr;      // 9 per pose, FP64
+    float *dinv32 = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
+    double *r64 = nullptr;       // mixed: FP64 true residual scratch
+    double *part = nullptr;      // reduction partials (pcg_common.h layout)
+    PcgStatus *status = nullptr;
+    PcgStatus *status_host = nullptr;  // pinned mirror
+    PcgStatus *status_ring = nullptr;  // pinned, 2 slots for look-ahead polling
+    double *scal = nullptr;      // small device scalars (diff etc.)
+    double *scal_host = nullptr; // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
+
+    // factorized FP32 operator of the mixed solver: per incidence entry k
+    // (pose i, incident edge e, side a; layout of xlist) the other pose
+    // xoth[k] and 8 FP32 edge factors frec[8k..8k+7]; xpos[2e+a] = k.
+    int32_t *xoth = nullptr, *xpos = nullptr;
+    float *frec = nullptr;
+
+    std::vector<void *> allocs;
+
+    template <typename T>
+    int alloc(T **p, int64_t count);
+    int upload(void *dst, const void *src, int64_t bytes);
+    int download(void *dst, const void *src, int64_t bytes);
+    ~DeviceGraph();
+};
+
+template <typename T>
+int DeviceGraph::alloc(T **ptr, int64_t count)
+{
+    size_t bytes_ = static_cast<size_t>(count > 0 ? count : 1) * sizeof(T);
+    void *mem = nullptr;
+    hipError_t e = hipMalloc(&mem, bytes_);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(bytes_) + "): " +
+                  hipGetErrorString(e));
+        return GSLAM_ERR_NOMEM;
+    }
+    allocs.push_back(mem);
+    bytes += static_cast<int64_t>(bytes_);
+    *ptr = static_cast<T *>(mem);
+    return GSLAM_OK;
+}
+
+// ---- assembly.hip ----
+// Evaluates every edge at the current poses and gathers Omega (BSR, FP64 and
+// FP32 copy) and xi.
+int launch_assembly(DeviceGraph &g);
+// BSR values -> reference CSR values (device to device).
+int launch_bsr_to_csr(DeviceGraph &g, const double *bval, double *csr_vals);
+int launch_csr_to_bsr(DeviceGraph &g, const double *csr_vals, double *bval);
+
+// ---- gj.hip ----
+int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                gslam_solve_stats &s);
+
+// ---- pcg.hip ----
+int pcg_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b,
+              double *x_out, gslam_solve_stats &s);
+int pose_update(DeviceGraph &g, double *diff_out);
+// y = Omega x (FP64, BSR) on the handle stream.
+int launch_spmv64(DeviceGraph &g, const double *x, double *y);
+// ||b - Omega x|| / ||b|| in FP64 (device reductions, one host sync).
+int true_rel_residual(DeviceGraph &g, const double *b, const double *x, double *out);
+
+}  // namespace gslam
+
+struct gslam_graph {
+    gslam::DeviceGraph g;
+};

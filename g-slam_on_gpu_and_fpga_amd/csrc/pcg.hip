@@ -1,0 +1,436 @@
+// Block-Jacobi preconditioned CG on the BSR Omega, FP64, fully device
+// resident.  Algorithm and stopping rule of pcg_solve_sparse
+// (src/cpu_pcg_fp64/MatrixOps.c:281-391): x0 = 0, r = b, z = D^-1 r, p = z;
+// per iteration q = A p, alpha = rz/pq (pq == 0 -> stop before updating),
+// x += alpha p, r -= alpha q, stop when ||r|| < tol ||b|| (checked after the
+// update), z = D^-1 r, beta = rz_new/rz_old, p = z + beta p.
+//
+// Two kernels per iteration, no host round trip and no atomics:
+//   A(k) "spmv":   sums the r.r / r.z partials B(k-1) left behind (every
+//                  workgroup redundantly, in the same fixed order, so all
+//                  agree bit for bit), applies the stopping test, forms
+//                  p = z + beta p_old on the fly for every gathered column
+//                  (double-buffered p), q = A p, and leaves p.q partials;
+//   B(k) "update": sums the p.q partials, alpha = rz/pq, updates x, r, z for
+//                  its own 3x3 block rows and leaves r.r / r.z partials.
+// Kernel boundaries order the partials; a grid-wide reduction through one
+// atomic counter costs ~15-35 us per iteration on MI355X at 1-2k
+// workgroups, the redundant re-read of <= 512 partials ~1 us.  Only
+// workgroup 0 writes the status record the host polls.
+#include "device_common.h"
+#include "pcg_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace gslam {
+
+constexpr int kLanesPerRow = 4;  // lanes cooperating on one 3x3 block row
+
+__device__ inline void inv3_cofactor(const double *a, double *inv)
+{
+    // MatrixOps.c:301-314: det == 0 is replaced by 1e-30
+    double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                 a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (det == 0.0) det = 1e-30;
+    inv[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+    inv[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    inv[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+    inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    inv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+    inv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+__global__ void __launch_bounds__(kThreads)
+pcg_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ bval,
+                 const double *__restrict__ b, int32_t n, int32_t max_iter,
+                 double *__restrict__ dinv, float *__restrict__ dinv32, double *__restrict__ x,
+                 double *__restrict__ r, double *__restrict__ z, double *__restrict__ p0,
+                 double *__restrict__ p1, double *__restrict__ part_out, PcgStatus *status)
+{
+    double v[2] = {0.0, 0.0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a[9], inv[9];
+        const double *blk = bval + 9 * static_cast<int64_t>(bdiag[i]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k] = blk[k];
+        inv3_cofactor(a, inv);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            dinv[9 * static_cast<int64_t>(i) + k] = inv[k];
+            if (dinv32) dinv32[9 * static_cast<int64_t>(i) + k] = static_cast<float>(inv[k]);
+        }
+        const int64_t o = 3 * static_cast<int64_t>(i);
+        const double b0 = b[o], b1 = b[o + 1], b2 = b[o + 2];
+        const double z0 = inv[0] * b0 + inv[1] * b1 + inv[2] * b2;
+        const double z1 = inv[3] * b0 + inv[4] * b1 + inv[5] * b2;
+        const double z2 = inv[6] * b0 + inv[7] * b1 + inv[8] * b2;
+        if (x) { x[o] = 0.0; x[o + 1] = 0.0; x[o + 2] = 0.0; }
+        if (r) { r[o] = b0; r[o + 1] = b1; r[o + 2] = b2; }
+        if (z) { z[o] = z0; z[o + 1] = z1; z[o + 2] = z2; }
+        if (p0) { p0[o] = 0.0; p0[o + 1] = 0.0; p0[o + 2] = 0.0; }
+        if (p1) { p1[o] = 0.0; p1[o + 1] = 0.0; p1[o + 2] = 0.0; }
+        v[0] += b0 * b0 + b1 * b1 + b2 * b2;   // ||r0||^2 = ||b||^2
+        v[1] += b0 * z0 + b1 * z1 + b2 * z2;
+    }
+    write_partials<2>(v, part_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        status->done = max_iter <= 0 ? 1 : 0;
+        status->converged = 0;
+        status->iters = 0;
+        status->flags = 0;
+        status->refreshes = 0;
+        status->rnorm = 0.0;
+        status->bnorm = 0.0;
+        status->tol = 0.0;
+    }
+}
+
+// A(k): stopping test on the partials of B(k-1), then q = A p with
+// p = z + beta p_old formed per gathered column; p.q partials.
+template <int G>
+__global__ void __launch_bounds__(kThreads)
+pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                const double *__restrict__ bval, const double *__restrict__ z,
+                const double *__restrict__ p_old, double *__restrict__ p_new,
+                double *__restrict__ q, int32_t n, int32_t k, double tol_rel,
+                const double *__restrict__ pr_cur, int32_t np_cur,
+                const double *__restrict__ pr_prev, int32_t np_prev,
+                double *__restrict__ pq_out, PcgStatus *status)
+{
+    if (status->done) return;
+    double cur[2];
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);  // rr_k, rz_k
+    double tol;
+    if (k == 0) {
+        tol = tol_rel * sqrt(cur[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->bnorm = sqrt(cur[0]);
+            status->tol = tol;
+            status->rnorm = sqrt(cur[0]);
+        }
+    } else {
+        tol = status->tol;
+        if (sqrt(cur[0]) < tol) {  // MatrixOps.c:366: checked after the update
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                status->done = 1;
+                status->converged = 1;
+                status->iters = k;
+                status->rnorm = sqrt(cur[0]);
+            }
+            return;
+        }
+    }
+    double beta = 0.0;
+    if (k > 0) {
+        double prev[2];
+        sum_partials<2>(pr_prev, np_prev, kPartStride, prev);
+        beta = cur[1] / prev[1];
+    }
+    constexpr int kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    double pq[1] = {0.0};
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < n) {
+            const int32_t b1 = brow_ptr[row + 1];
+            for (int32_t bb = brow_ptr[row] + lane; bb < b1; bb += G) {
+                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+                const double pj0 = z[j] + beta * p_old[j];
+                const double pj1 = z[j + 1] + beta * p_old[j + 1];
+                const double pj2 = z[j + 2] + beta * p_old[j + 2];
+                const double *v = bval + 9 * static_cast<int64_t>(bb);
+                y0 += v[0] * pj0 + v[1] * pj1 + v[2] * pj2;
+                y1 += v[3] * pj0 + v[4] * pj1 + v[5] * pj2;
+                y2 += v[6] * pj0 + v[7] * pj1 + v[8] * pj2;
+            }
+        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
+        if (row < n && lane == 0) {
+            const int64_t o = 3 * row;
+            const double pi0 = z[o] + beta * p_old[o];
+            const double pi1 = z[o + 1] + beta * p_old[o + 1];
+            const double pi2 = z[o + 2] + beta * p_old[o + 2];
+            p_new[o] = pi0; p_new[o + 1] = pi1; p_new[o + 2] = pi2;
+            q[o] = y0; q[o + 1] = y1; q[o + 2] = y2;
+            pq[0] += pi0 * y0 + pi1 * y1 + pi2 * y2;
+        }
+    }
+    write_partials<1>(pq, pq_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        status->iters = k;
+        status->rnorm = sqrt(cur[0]);
+    }
+}
+
+// B(k): alpha from the p.q partials, x/r/z update, r.r / r.z partials.
+__global__ void __launch_bounds__(kThreads)
+pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
+                  const double *__restrict__ dinv, double *__restrict__ x,
+                  double *__restrict__ r, double *__restrict__ z, int32_t n,
+                  const double *__restrict__ pq_in, int32_t np_pq,
+                  const double *__restrict__ pr_cur, int32_t np_cur,
+                  double *__restrict__ pr_out, PcgStatus *status)
+{
+    if (status->done) return;
+    double pq[1], cur[2];
+    sum_partials<1>(pq_in, np_pq, kPartStride, pq);
+    if (pq[0] == 0.0) {  // MatrixOps.c:357
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->done = 1;
+            status->flags |= 1;
+        }
+        return;
+    }
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    const double alpha = cur[1] / pq[0];
+    double v[2] = {0.0, 0.0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t o = 3 * static_cast<int64_t>(i);
+        double rr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[o + c] += alpha * p[o + c];
+            rr[c] = r[o + c] - alpha * q[o + c];
+            r[o + c] = rr[c];
+        }
+        const double *d = dinv + 9 * static_cast<int64_t>(i);
+        double zz[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) zz[c] = d[3 * c] * rr[0] + d[3 * c + 1] * rr[1] + d[3 * c + 2] * rr[2];
+        z[o] = zz[0]; z[o + 1] = zz[1]; z[o + 2] = zz[2];
+        v[0] += rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
+        v[1] += rr[0] * zz[0] + rr[1] * zz[1] + rr[2] * zz[2];
+    }
+    write_partials<2>(v, pr_out, kPartStride);
+}
+
+// After the last launched iteration (or at a poll): the stopping test on the
+// newest partials, for the host.
+__global__ void __launch_bounds__(kThreads)
+pcg_finalize_kernel(const double *__restrict__ pr_cur, int32_t np_cur, int32_t k,
+                    int32_t max_iter, PcgStatus *status)
+{
+    if (status->done || k == 0) return;
+    double cur[2];
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    if (threadIdx.x == 0) {
+        status->iters = k;
+        status->rnorm = sqrt(cur[0]);
+        if (sqrt(cur[0]) < status->tol) {
+            status->done = 1;
+            status->converged = 1;
+        } else if (k >= max_iter) {
+            status->done = 1;
+        }
+    }
+}
+
+// y = A x (FP64) and, if b != NULL, ||b - y||^2 / ||b||^2 partials.
+__global__ void __launch_bounds__(kThreads)
+spmv64_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+              const double *__restrict__ bval, const double *__restrict__ x,
+              const double *__restrict__ b, double *__restrict__ y, int32_t n,
+              double *__restrict__ part_out)
+{
+    constexpr int G = kLanesPerRow, kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    double s[2] = {0.0, 0.0};
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < n) {
+            const int32_t b1 = brow_ptr[row + 1];
+            for (int32_t bb = brow_ptr[row] + lane; bb < b1; bb += G) {
+                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+                const double *v = bval + 9 * static_cast<int64_t>(bb);
+                y0 += v[0] * x[j] + v[1] * x[j + 1] + v[2] * x[j + 2];
+                y1 += v[3] * x[j] + v[4] * x[j + 1] + v[5] * x[j + 2];
+                y2 += v[6] * x[j] + v[7] * x[j + 1] + v[8] * x[j + 2];
+            }
+        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
+        if (row < n && lane == 0) {
+            const int64_t o = 3 * row;
+            if (y) { y[o] = y0; y[o + 1] = y1; y[o + 2] = y2; }
+            if (b) {
+                const double d0 = b[o] - y0, d1 = b[o + 1] - y1, d2 = b[o + 2] - y2;
+                s[0] += d0 * d0 + d1 * d1 + d2 * d2;
+                s[1] += b[o] * b[o] + b[o + 1] * b[o + 1] + b[o + 2] * b[o + 2];
+            }
+        }
+    }
+    if (b) write_partials<2>(s, part_out, kPartStride);
+}
+
+__global__ void __launch_bounds__(kThreads)
+ratio_kernel(const double *__restrict__ part, int32_t np, double *__restrict__ out)
+{
+    double t[2];
+    sum_partials<2>(part, np, kPartStride, t);
+    if (threadIdx.x == 0) *out = t[1] > 0.0 ? sqrt(t[0]) / sqrt(t[1]) : sqrt(t[0]);
+}
+
+// poses += dx (GSLAM.c:98-102), optional theta wrap; ||dx||^2 partials.
+__global__ void __launch_bounds__(kThreads)
+pose_update_kernel(double *__restrict__ poses, const double *__restrict__ dx, int32_t n,
+                   int wrap, double *__restrict__ part_out)
+{
+    double s[1] = {0.0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t o = 3 * static_cast<int64_t>(i);
+        const double d0 = dx[o], d1 = dx[o + 1], d2 = dx[o + 2];
+        poses[o] = poses[o] + d0;
+        poses[o + 1] = poses[o + 1] + d1;
+        double th = poses[o + 2] + d2;
+        if (wrap) th = wrap_pi(th);
+        poses[o + 2] = th;
+        s[0] += d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    write_partials<1>(s, part_out, kPartStride);
+}
+
+__global__ void __launch_bounds__(kThreads)
+diff_kernel(const double *__restrict__ part, int32_t np, int32_t n, double *__restrict__ out)
+{
+    double t[1];
+    sum_partials<1>(part, np, kPartStride, t);
+    if (threadIdx.x == 0) *out = sqrt(t[0]) / sqrt(3.0 * static_cast<double>(n));
+}
+
+int launch_spmv64(DeviceGraph &g, const double *x, double *y)
+{
+    hipLaunchKernelGGL(spmv64_kernel, dim3(grid_for(g.h.n, kLanesPerRow)), dim3(kThreads), 0,
+                       g.stream, g.brow_ptr, g.bcol, g.bval, x, nullptr, y, g.h.n, g.part);
+    GSLAM_HIP(hipGetLastError());
+    return GSLAM_OK;
+}
+
+int enqueue_true_residual(DeviceGraph &g, const double *b, const double *x, double *dst)
+{
+    const unsigned int gr = grid_for(g.h.n, kLanesPerRow);
+    hipLaunchKernelGGL(spmv64_kernel, dim3(gr), dim3(kThreads), 0, g.stream, g.brow_ptr, g.bcol,
+                       g.bval, x, b, nullptr, g.h.n, g.part + kPartOffResid);
+    hipLaunchKernelGGL(ratio_kernel, dim3(1), dim3(kThreads), 0, g.stream,
+                       g.part + kPartOffResid, static_cast<int32_t>(gr), dst);
+    GSLAM_HIP(hipGetLastError());
+    return GSLAM_OK;
+}
+
+int true_rel_residual(DeviceGraph &g, const double *b, const double *x, double *out)
+{
+    GSLAM_TRY(enqueue_true_residual(g, b, x, g.scal + 2));
+    GSLAM_HIP(hipMemcpyAsync(g.scal_host + 2, g.scal + 2, sizeof(double), hipMemcpyDeviceToHost,
+                             g.stream));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    *out = g.scal_host[2];
+    return GSLAM_OK;
+}
+
+int pose_update(DeviceGraph &g, double *diff_out)
+{
+    const int32_t n = g.h.n;
+    const unsigned int gr = grid_for(n, 1);
+    hipLaunchKernelGGL(pose_update_kernel, dim3(gr), dim3(kThreads), 0, g.stream, g.poses, g.dx,
+                       n, g.wrap, g.part + kPartOffResid);
+    hipLaunchKernelGGL(diff_kernel, dim3(1), dim3(kThreads), 0, g.stream, g.part + kPartOffResid,
+                       static_cast<int32_t>(gr), n, g.scal + 1);
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipMemcpyAsync(g.scal_host + 1, g.scal + 1, sizeof(double), hipMemcpyDeviceToHost,
+                             g.stream));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    if (diff_out) *diff_out = g.scal_host[1];
+    return GSLAM_OK;
+}
+
+int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                   gslam_solve_stats &s)
+{
+    const int32_t n = g.h.n;
+    const unsigned int gs = grid_for(n, kLanesPerRow), gu = grid_for(n, 1);
+    double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
+    double *PQ = g.part + kPartOffPQ;
+    PcgStatus *st = g.status;
+    GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    hipLaunchKernelGGL(pcg_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval, b,
+                       n, o.max_iter, g.dinv, static_cast<float *>(nullptr), x_out, g.r, g.z,
+                       g.p[0], g.p[1], PR[0], st);
+    GSLAM_HIP(hipGetLastError());
+    int32_t k = 0;
+    auto enqueue_iters = [&](int32_t m) {
+        for (int32_t c = 0; c < m; ++c, ++k) {
+            double *pold = g.p[k & 1], *pnew = g.p[(k + 1) & 1];
+            hipLaunchKernelGGL(pcg_spmv_kernel<kLanesPerRow>, dim3(gs), dim3(kThreads), 0, g.stream,
+                               g.brow_ptr, g.bcol, g.bval, g.z, pold, pnew, g.q, n, k, o.tol,
+                               PR[k & 1], static_cast<int32_t>(gu), PR[(k + 1) & 1],
+                               static_cast<int32_t>(gu), PQ, st);
+            hipLaunchKernelGGL(pcg_update_kernel, dim3(gu), dim3(kThreads), 0, g.stream, pnew, g.q,
+                               g.dinv, x_out, g.r, g.z, n, PQ, static_cast<int32_t>(gs), PR[k & 1],
+                               static_cast<int32_t>(gu), PR[(k + 1) & 1], st);
+        }
+        hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(kThreads), 0, g.stream, PR[k & 1],
+                           static_cast<int32_t>(gu), k, o.max_iter, st);
+    };
+    GSLAM_TRY(run_chunked(g, o, enqueue_iters, k));
+    GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
+    GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    const PcgStatus &ps = *g.status_host;
+    s.iters = ps.iters;
+    s.converged = ps.converged;
+    s.flags = ps.flags;
+    s.reliable_updates = 0;
+    s.rel_res = ps.bnorm > 0 ? ps.rnorm / ps.bnorm : ps.rnorm;
+    s.ms = ms;
+    GSLAM_TRY(true_rel_residual(g, b, x_out, &s.true_rel_res));
+    return GSLAM_OK;
+}
+
+// Chunked launch loop shared by the solvers: enqueue chunks of iterations,
+// keep one chunk in flight ahead of the host's status check so the GPU never
+// idles on a synchronisation, stop when the device reports done or the cap.
+int run_chunked(DeviceGraph &g, const gslam_solve_opts &o,
+                const std::function<void(int32_t)> &enqueue, int32_t &k)
+{
+    int32_t chunk = 8;
+    const int32_t cap = std::max(1, o.check_every > 0 ? o.check_every : 256);
+    int slot = 0;
+    bool pending = false;
+    while (k < o.max_iter) {
+        const int32_t m = std::min(chunk, o.max_iter - k);
+        enqueue(m);
+        GSLAM_HIP(hipGetLastError());
+        GSLAM_HIP(hipMemcpyAsync(g.status_ring + slot, g.status, sizeof(PcgStatus),
+                                 hipMemcpyDeviceToHost, g.stream));
+        GSLAM_HIP(hipEventRecord(g.ev_poll[slot], g.stream));
+        if (pending) {
+            GSLAM_HIP(hipEventSynchronize(g.ev_poll[slot ^ 1]));
+            if (g.status_ring[slot ^ 1].done) break;
+        }
+        pending = true;
+        slot ^= 1;
+        chunk = std::min(chunk * 2, cap);
+    }
+    return GSLAM_OK;
+}
+
+int pcg_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+              gslam_solve_stats &s)
+{
+    if (o.solver == GSLAM_SOLVER_PCG_MIXED) return pcg_solve_mixed(g, o, b, x_out, s);
+    return pcg_solve_fp64(g, o, b, x_out, s);
+}
+
+}  // namespace gslam

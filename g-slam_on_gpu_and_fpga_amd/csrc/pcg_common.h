@@ -1,0 +1,75 @@
+// Pieces shared by the PCG solvers: the host-visible status record, the
+// partial-sum layout and the redundant (atomic-free) grid reductions.
+#pragma once
+
+#include <functional>
+
+#include "device_common.h"
+#include "gslam_internal.h"
+
+namespace gslam {
+
+// Streaming kernels use at most kMaxGrid workgroups (grid-stride loops), so a
+// consumer re-reads at most kMaxGrid partials per reduced quantity.
+constexpr int kMaxGrid = 512;
+constexpr int kPartStride = kMaxGrid;
+// Offsets (doubles) into DeviceGraph::part.
+constexpr int kPartOffPR0 = 0 * kPartStride;    // r.r, r.z of even iterations (2 arrays)
+constexpr int kPartOffPR1 = 2 * kPartStride;    // r.r, r.z of odd iterations (2 arrays)
+constexpr int kPartOffPQ = 4 * kPartStride;     // p.q (1 array)
+constexpr int kPartOffResid = 6 * kPartStride;  // residual / update norms (2 arrays)
+constexpr int kPartTotal = 8 * kPartStride;
+
+inline unsigned int grid_for(int64_t rows, int lanes)
+{
+    int64_t g = (rows * lanes + kThreads - 1) / kThreads;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return static_cast<unsigned int>(g);
+}
+
+// Block sums of K values, written by thread 0 to out[k * stride + blockIdx.x].
+template <int K>
+__device__ inline void write_partials(double (&v)[K], double *out, int stride)
+{
+    __shared__ double lds[K][kThreads / kWave];
+    block_sum<K, kThreads>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k * stride + blockIdx.x] = v[k];
+    }
+}
+
+// Sum of np partials per quantity in a fixed order; every thread of every
+// workgroup obtains the same bits.
+template <int K>
+__device__ inline void sum_partials(const double *part, int np, int stride, double (&out)[K])
+{
+    __shared__ double lds[K][kThreads / kWave];
+    __shared__ double res[K];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = 0.0;
+        for (int i = threadIdx.x; i < np; i += kThreads) v[k] += part[k * stride + i];
+    }
+    block_sum<K, kThreads>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) res[k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = res[k];
+}
+
+int run_chunked(DeviceGraph &g, const gslam_solve_opts &o,
+                const std::function<void(int32_t)> &enqueue, int32_t &k);
+int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                   gslam_solve_stats &s);
+int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                    gslam_solve_stats &s);
+// Enqueues ||b - A x|| / ||b|| (FP64) into the device scalar dst.
+int enqueue_true_residual(DeviceGraph &g, const double *b, const double *x, double *dst);
+
+}  // namespace gslam

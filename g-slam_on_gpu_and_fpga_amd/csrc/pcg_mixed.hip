@@ -1,0 +1,415 @@
+// Mixed-precision block-Jacobi PCG: FP32 inner iterations, FP64 refinement
+// on the true residual.
+//
+// Inner iterations: FP32 r, z, p, q, FP32 block-Jacobi inverses and an FP32
+// operator; the iterate x is accumulated in FP64 and dot products are summed
+// in FP64.  Every refresh_every iterations an FP64 pass recomputes the true
+// residual r = b - Omega x with the FP64 BSR Omega, rounds it into the FP32
+// recurrence and restarts z and beta from it while keeping the search
+// direction p (reliable updates, Sleijpen & van der Vorst 1996).
+// Convergence is declared only on that FP64 true residual.
+//
+// The FP32 operator is applied in factorized form,
+//     (Omega p)_i = anchor_i p_i + sum_{edges e at i} J_a^T W_e (J_a p_i + J_b p_o),
+// from 8 FP32 factors per edge end (frec) written by the assembly kernels.
+// Forming the edge residual-like term J_a p_i + J_b p_o (a difference of
+// neighbouring values) before scaling keeps the near-null rigid-motion and
+// low-frequency modes of long pose chains representable in FP32; an FP32
+// copy of the assembled blocks loses them (entry-wise rounding of ~1e3
+// entries whose combinations cancel to ~1e-4) and stalls at a true relative
+// residual of ~1e-8 on the N=10k grid-city graph.
+//
+// The reference's only reduced-precision PCG (src/gpu_pcg_fp32/
+// device_graph.cu:611-692) casts the assembled system to FP32 and never
+// refines.
+#include "device_common.h"
+#include "pcg_common.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace gslam {
+
+constexpr int kLanesMixed = 4;
+
+__global__ void __launch_bounds__(kThreads)
+mixed_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ bval,
+                   const double *__restrict__ b, int32_t n, int32_t max_iter,
+                   float *__restrict__ dinv32, double *__restrict__ x, float *__restrict__ r,
+                   float *__restrict__ z, float *__restrict__ p0, float *__restrict__ p1,
+                   double *__restrict__ part_out, PcgStatus *status)
+{
+    double v[2] = {0.0, 0.0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double *a = bval + 9 * static_cast<int64_t>(bdiag[i]);
+        double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+        if (det == 0.0) det = 1e-30;
+        float fi[9];
+        fi[0] = static_cast<float>((a[4] * a[8] - a[5] * a[7]) / det);
+        fi[1] = static_cast<float>((a[2] * a[7] - a[1] * a[8]) / det);
+        fi[2] = static_cast<float>((a[1] * a[5] - a[2] * a[4]) / det);
+        fi[3] = static_cast<float>((a[5] * a[6] - a[3] * a[8]) / det);
+        fi[4] = static_cast<float>((a[0] * a[8] - a[2] * a[6]) / det);
+        fi[5] = static_cast<float>((a[2] * a[3] - a[0] * a[5]) / det);
+        fi[6] = static_cast<float>((a[3] * a[7] - a[4] * a[6]) / det);
+        fi[7] = static_cast<float>((a[1] * a[6] - a[0] * a[7]) / det);
+        fi[8] = static_cast<float>((a[0] * a[4] - a[1] * a[3]) / det);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dinv32[9 * static_cast<int64_t>(i) + k] = fi[k];
+        const int64_t o = 3 * static_cast<int64_t>(i);
+        const float b0 = static_cast<float>(b[o]), b1 = static_cast<float>(b[o + 1]),
+                    b2 = static_cast<float>(b[o + 2]);
+        const float z0 = fi[0] * b0 + fi[1] * b1 + fi[2] * b2;
+        const float z1 = fi[3] * b0 + fi[4] * b1 + fi[5] * b2;
+        const float z2 = fi[6] * b0 + fi[7] * b1 + fi[8] * b2;
+        x[o] = 0.0; x[o + 1] = 0.0; x[o + 2] = 0.0;
+        r[o] = b0; r[o + 1] = b1; r[o + 2] = b2;
+        z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+        p0[o] = 0.f; p0[o + 1] = 0.f; p0[o + 2] = 0.f;
+        p1[o] = 0.f; p1[o + 1] = 0.f; p1[o + 2] = 0.f;
+        v[0] += b[o] * b[o] + b[o + 1] * b[o + 1] + b[o + 2] * b[o + 2];
+        v[1] += static_cast<double>(b0) * z0 + static_cast<double>(b1) * z1 +
+                static_cast<double>(b2) * z2;
+    }
+    write_partials<2>(v, part_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        status->done = max_iter <= 0 ? 1 : 0;
+        status->converged = 0;
+        status->iters = 0;
+        status->flags = 0;
+        status->refreshes = 0;
+        status->rnorm = 0.0;
+        status->bnorm = 0.0;
+        status->tol = 0.0;
+    }
+}
+
+// y_i contribution of one edge end; see the factor layout in assembly.hip.
+// type 0: landmark pair (B1 = [-I | b1], B2 = [I | b2], W 2x2)
+// type 1: motion (J1 = -F, J2 = I, W = lambda Om 3x3)
+// type 2: SE(2) between (Ji, Jj of the rotation c, s and ga, gb; W diagonal)
+__device__ inline void edge_apply(int type, int side, const float4 f0, const float4 f1,
+                                  const float pi[3], const float po[3], float y[3])
+{
+    if (type == 2) {
+        const float c = f0.x, s = f0.y, ga = f0.z, gb = f0.w;
+        if (side == 0) {  // pose i is the "from" end
+            const float d0 = po[0] - pi[0], d1 = po[1] - pi[1];
+            const float u0 = c * d0 + s * d1 + ga * pi[2];
+            const float u1 = -s * d0 + c * d1 + gb * pi[2];
+            const float u2 = po[2] - pi[2];
+            const float w0 = f1.x * u0, w1 = f1.y * u1, w2 = f1.z * u2;
+            y[0] += -c * w0 + s * w1;
+            y[1] += -s * w0 - c * w1;
+            y[2] += ga * w0 + gb * w1 - w2;
+        } else {          // pose i is the "to" end, o the "from" end
+            const float d0 = pi[0] - po[0], d1 = pi[1] - po[1];
+            const float u0 = c * d0 + s * d1 + ga * po[2];
+            const float u1 = -s * d0 + c * d1 + gb * po[2];
+            const float u2 = pi[2] - po[2];
+            const float w0 = f1.x * u0, w1 = f1.y * u1, w2 = f1.z * u2;
+            y[0] += c * w0 - s * w1;
+            y[1] += s * w0 + c * w1;
+            y[2] += w2;
+        }
+    } else if (type == 0) {
+        const float b1x = f0.x, b1y = f0.y, b2x = f0.z, b2y = f0.w;
+        const float w00 = f1.x, w01 = f1.y, w11 = f1.z;
+        // u = B1 p_(obs1 pose) + B2 p_(obs2 pose)
+        const float *p1 = side == 0 ? pi : po, *p2 = side == 0 ? po : pi;
+        const float u0 = (p2[0] - p1[0]) + b1x * p1[2] + b2x * p2[2];
+        const float u1 = (p2[1] - p1[1]) + b1y * p1[2] + b2y * p2[2];
+        const float w0 = w00 * u0 + w01 * u1, w1 = w01 * u0 + w11 * u1;
+        if (side == 0) {
+            y[0] -= w0;
+            y[1] -= w1;
+            y[2] += b1x * w0 + b1y * w1;
+        } else {
+            y[0] += w0;
+            y[1] += w1;
+            y[2] += b2x * w0 + b2y * w1;
+        }
+    } else {
+        const float F02 = f0.x, F12 = f0.y;
+        const float w00 = f0.z, w01 = f0.w, w02 = f1.x, w11 = f1.y, w12 = f1.z, w22 = f1.w;
+        // u = -F p_(t1) + p_(t2)
+        const float *p1 = side == 0 ? pi : po, *p2 = side == 0 ? po : pi;
+        const float u0 = (p2[0] - p1[0]) - F02 * p1[2];
+        const float u1 = (p2[1] - p1[1]) - F12 * p1[2];
+        const float u2 = p2[2] - p1[2];
+        const float w0 = w00 * u0 + w01 * u1 + w02 * u2;
+        const float w1 = w01 * u0 + w11 * u1 + w12 * u2;
+        const float w2 = w02 * u0 + w12 * u1 + w22 * u2;
+        if (side == 0) {
+            y[0] -= w0;
+            y[1] -= w1;
+            y[2] -= F02 * w0 + F12 * w1 + w2;
+        } else {
+            y[0] += w0;
+            y[1] += w1;
+            y[2] += w2;
+        }
+    }
+}
+
+// A(k): stopping test (true residual slots only), beta, q = Omega_32 p with
+// p = z + beta p_old on the fly, p.q partials.
+template <int G>
+__global__ void __launch_bounds__(kThreads)
+mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ xlist,
+                  const int32_t *__restrict__ xoth, const float4 *__restrict__ frec,
+                  int64_t n_obs_e, int64_t n_mot_e, float anchor, const float *__restrict__ z,
+                  const float *__restrict__ p_old, float *__restrict__ p_new,
+                  float *__restrict__ q, int32_t n, int32_t k, int32_t is_true, double tol_rel,
+                  const double *__restrict__ pr_cur, int32_t np_cur,
+                  const double *__restrict__ pr_prev, int32_t np_prev,
+                  double *__restrict__ pq_out, PcgStatus *status)
+{
+    if (status->done) return;
+    double cur[2];
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    if (k == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->bnorm = sqrt(cur[0]);
+            status->tol = tol_rel * sqrt(cur[0]);
+            status->rnorm = sqrt(cur[0]);
+        }
+    } else if (is_true) {
+        const double tol = status->tol;
+        if (sqrt(cur[0]) < tol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                status->done = 1;
+                status->converged = 1;
+                status->iters = k;
+                status->rnorm = sqrt(cur[0]);
+            }
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) status->rnorm = sqrt(cur[0]);
+    }
+    float beta = 0.f;
+    if (k > 0) {
+        double prev[2];
+        sum_partials<2>(pr_prev, np_prev, kPartStride, prev);
+        beta = static_cast<float>(cur[1] / prev[1]);
+    }
+    constexpr int kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    double pq[1] = {0.0};
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        float y[3] = {0.f, 0.f, 0.f};
+        float pi[3] = {0.f, 0.f, 0.f};
+        if (row < n) {
+            const int64_t o = 3 * row;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pi[c] = z[o + c] + beta * p_old[o + c];
+            const int32_t e1 = xptr[row + 1];
+            for (int32_t kk = xptr[row] + lane; kk < e1; kk += G) {
+                const int32_t ent = xlist[kk];
+                const int64_t e = ent >> 1;
+                const int side = ent & 1;
+                const int type = e < n_obs_e ? 0 : (e < n_obs_e + n_mot_e ? 1 : 2);
+                const int64_t oo = 3 * static_cast<int64_t>(xoth[kk]);
+                float po[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) po[c] = z[oo + c] + beta * p_old[oo + c];
+                edge_apply(type, side, frec[2 * kk], frec[2 * kk + 1], pi, po, y);
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            y[0] += __shfl_xor(y[0], off, G);
+            y[1] += __shfl_xor(y[1], off, G);
+            y[2] += __shfl_xor(y[2], off, G);
+        }
+        if (row < n && lane == 0) {
+            if (row == 0) {
+                y[0] += anchor * pi[0];
+                y[1] += anchor * pi[1];
+                y[2] += anchor * pi[2];
+            }
+            const int64_t o = 3 * row;
+            p_new[o] = pi[0]; p_new[o + 1] = pi[1]; p_new[o + 2] = pi[2];
+            q[o] = y[0]; q[o + 1] = y[1]; q[o + 2] = y[2];
+            pq[0] += static_cast<double>(pi[0]) * y[0] + static_cast<double>(pi[1]) * y[1] +
+                     static_cast<double>(pi[2]) * y[2];
+        }
+    }
+    write_partials<1>(pq, pq_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) status->iters = k;
+}
+
+__global__ void __launch_bounds__(kThreads)
+mixed_update_kernel(const float *__restrict__ p, const float *__restrict__ q,
+                    const float *__restrict__ dinv32, double *__restrict__ x,
+                    float *__restrict__ r, float *__restrict__ z, int32_t n,
+                    const double *__restrict__ pq_in, int32_t np_pq,
+                    const double *__restrict__ pr_cur, int32_t np_cur,
+                    double *__restrict__ pr_out, PcgStatus *status)
+{
+    if (status->done) return;
+    double pq[1], cur[2];
+    sum_partials<1>(pq_in, np_pq, kPartStride, pq);
+    if (pq[0] == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->done = 1;
+            status->flags |= 1;
+        }
+        return;
+    }
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    const double alpha = cur[1] / pq[0];
+    const float af = static_cast<float>(alpha);
+    double v[2] = {0.0, 0.0};
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t o = 3 * static_cast<int64_t>(i);
+        float rr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            x[o + c] += alpha * static_cast<double>(p[o + c]);
+            rr[c] = r[o + c] - af * q[o + c];
+            r[o + c] = rr[c];
+        }
+        const float *d = dinv32 + 9 * static_cast<int64_t>(i);
+        float zz[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) zz[c] = d[3 * c] * rr[0] + d[3 * c + 1] * rr[1] + d[3 * c + 2] * rr[2];
+        z[o] = zz[0]; z[o + 1] = zz[1]; z[o + 2] = zz[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[0] += static_cast<double>(rr[c]) * rr[c];
+            v[1] += static_cast<double>(rr[c]) * zz[c];
+        }
+    }
+    write_partials<2>(v, pr_out, kPartStride);
+}
+
+// Refresh: r = b - Omega x in FP64 (BSR), rounded into the FP32 recurrence,
+// z = D^-1 r; overwrites the r.r / r.z partials the next A(k) reads.
+__global__ void __launch_bounds__(kThreads)
+mixed_refresh_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                     const double *__restrict__ bval, const float *__restrict__ dinv32,
+                     const double *__restrict__ b, const double *__restrict__ x,
+                     float *__restrict__ r, float *__restrict__ z, int32_t n,
+                     double *__restrict__ pr_out, PcgStatus *status)
+{
+    if (status->done) return;
+    constexpr int G = kLanesMixed, kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    double s[2] = {0.0, 0.0};
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < n) {
+            const int32_t b1 = brow_ptr[row + 1];
+            for (int32_t bb = brow_ptr[row] + lane; bb < b1; bb += G) {
+                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+                const double *v = bval + 9 * static_cast<int64_t>(bb);
+                y0 += v[0] * x[j] + v[1] * x[j + 1] + v[2] * x[j + 2];
+                y1 += v[3] * x[j] + v[4] * x[j + 1] + v[5] * x[j + 2];
+                y2 += v[6] * x[j] + v[7] * x[j + 1] + v[8] * x[j + 2];
+            }
+        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
+        if (row < n && lane == 0) {
+            const int64_t o = 3 * row;
+            const double r0 = b[o] - y0, r1 = b[o + 1] - y1, r2 = b[o + 2] - y2;
+            const float f0 = static_cast<float>(r0), f1 = static_cast<float>(r1),
+                        f2 = static_cast<float>(r2);
+            const float *d = dinv32 + 9 * row;
+            const float z0 = d[0] * f0 + d[1] * f1 + d[2] * f2;
+            const float z1 = d[3] * f0 + d[4] * f1 + d[5] * f2;
+            const float z2 = d[6] * f0 + d[7] * f1 + d[8] * f2;
+            r[o] = f0; r[o + 1] = f1; r[o + 2] = f2;
+            z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+            s[0] += r0 * r0 + r1 * r1 + r2 * r2;
+            s[1] += static_cast<double>(f0) * z0 + static_cast<double>(f1) * z1 +
+                    static_cast<double>(f2) * z2;
+        }
+    }
+    write_partials<2>(s, pr_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) status->refreshes += 1;
+}
+
+__global__ void __launch_bounds__(kThreads)
+mixed_finalize_kernel(const double *__restrict__ pr_cur, int32_t np_cur, int32_t k,
+                      PcgStatus *status)
+{
+    double cur[2];
+    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    if (threadIdx.x == 0 && !status->done) {
+        status->iters = k;
+        status->rnorm = sqrt(cur[0]);
+        status->done = 1;
+        status->converged = sqrt(cur[0]) < status->tol ? 1 : 0;
+    }
+}
+
+int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                    gslam_solve_stats &s)
+{
+    const int32_t n = g.h.n;
+    const unsigned int gs = grid_for(n, kLanesMixed), gu = grid_for(n, 1);
+    const int32_t every = o.refresh_every > 0 ? o.refresh_every : 64;
+    double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
+    double *PQ = g.part + kPartOffPQ;
+    PcgStatus *st = g.status;
+    const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
+    GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
+                       b, n, o.max_iter, g.dinv32, x_out, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
+                       st);
+    GSLAM_HIP(hipGetLastError());
+    int32_t k = 0;
+    auto refresh = [&]() {
+        hipLaunchKernelGGL(mixed_refresh_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.brow_ptr,
+                           g.bcol, g.bval, g.dinv32, b, x_out, g.r32, g.z32, n, PR[k & 1], st);
+    };
+    auto enqueue_iters = [&](int32_t m) {
+        for (int32_t c = 0; c < m; ++c, ++k) {
+            int32_t is_true = (k == 0) ? 1 : 0;
+            if (k > 0 && k % every == 0) {
+                refresh();
+                is_true = 1;
+            }
+            float *pold = g.p32[k & 1], *pnew = g.p32[(k + 1) & 1];
+            hipLaunchKernelGGL(mixed_spmv_kernel<kLanesMixed>, dim3(gs), dim3(kThreads), 0, g.stream,
+                               g.xptr, g.xlist, g.xoth, frec, g.h.n_obs_e, g.h.n_mot_e,
+                               static_cast<float>(g.anchor), g.z32, pold, pnew, g.q32, n, k,
+                               is_true, o.tol, PR[k & 1], static_cast<int32_t>(gu),
+                               PR[(k + 1) & 1], static_cast<int32_t>(gu), PQ, st);
+            hipLaunchKernelGGL(mixed_update_kernel, dim3(gu), dim3(kThreads), 0, g.stream, pnew,
+                               g.q32, g.dinv32, x_out, g.r32, g.z32, n, PQ,
+                               static_cast<int32_t>(gs), PR[k & 1], static_cast<int32_t>(gu),
+                               PR[(k + 1) & 1], st);
+        }
+    };
+    GSLAM_TRY(run_chunked(g, o, enqueue_iters, k));
+    // final true residual (no-op if an earlier refresh already converged)
+    refresh();
+    hipLaunchKernelGGL(mixed_finalize_kernel, dim3(1), dim3(kThreads), 0, g.stream, PR[k & 1],
+                       static_cast<int32_t>(gu), k, st);
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
+    GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    const PcgStatus &ps = *g.status_host;
+    s.iters = ps.iters;
+    s.converged = ps.converged;
+    s.flags = ps.flags;
+    s.reliable_updates = ps.refreshes;
+    s.rel_res = ps.bnorm > 0 ? ps.rnorm / ps.bnorm : ps.rnorm;
+    s.ms = ms;
+    GSLAM_TRY(true_rel_residual(g, b, x_out, &s.true_rel_res));
+    return GSLAM_OK;
+}
+
+}  // namespace gslam

@@ -1,0 +1,341 @@
+"""Python binding of libgslam (ctypes over the C-ABI in include/gslam.h).
+
+The binding is a thin mirror of the C entry points: a `Graph` owns one
+`gslam_graph` handle; every call goes to the HIP library.  There is no CPU
+fallback: if libgslam.so is missing or fails to load, `GSlamError` is raised.
+
+    g = Graph.from_log("cityTrees800.txt", lam=0.1)     # reference .log.txt model
+    for _ in range(10):
+        stats, diff = g.gn_step(solver="pcg64", tol=1e-6, max_iter=500)
+    poses = g.get_poses()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("GSLAM_LIB", os.path.join(PKG_DIR, "libgslam.so"))
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_STATE, ERR_TOO_LARGE, ERR_NOT_CONVERGED = (
+    -1, -2, -3, -4, -5, -6, -7)
+SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3}
+
+
+class GSlamError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("n_poses", C.c_int32), ("poses", C.c_void_p),
+        ("motion", C.c_int32), ("u_nu", C.c_void_p), ("u_omega", C.c_void_p),
+        ("delta", C.c_double), ("lambda_", C.c_double), ("mns", C.c_double * 4),
+        ("n_obs", C.c_int32), ("z_step", C.c_void_p), ("z_lm", C.c_void_p),
+        ("z_range", C.c_void_p), ("z_bearing", C.c_void_p), ("pair_order", C.c_int32),
+        ("snr", C.c_double * 2),
+        ("n_between", C.c_int64), ("be_i", C.c_void_p), ("be_j", C.c_void_p),
+        ("be_meas", C.c_void_p), ("be_info", C.c_void_p),
+        ("anchor", C.c_double), ("wrap", C.c_int32)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("n_poses", C.c_int32), ("n_edges", C.c_int64), ("n_obs_edges", C.c_int64),
+                ("n_motion_edges", C.c_int64), ("n_between_edges", C.c_int64),
+                ("n_blocks", C.c_int64), ("nnz", C.c_int64), ("device_bytes", C.c_int64)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("solver", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
+                ("refine_sweeps", C.c_int32), ("refresh_every", C.c_int32),
+                ("check_every", C.c_int32)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("reliable_updates", C.c_int32),
+                ("flags", C.c_int32), ("rel_res", C.c_double), ("true_rel_res", C.c_double),
+                ("ms", C.c_double)]
+
+
+_LIB = None
+
+# every symbol of include/gslam.h and include/gslam_compat.h
+EXPORTS = [
+    "gslam_default_opts", "gslam_default_desc", "gslam_create", "gslam_destroy",
+    "gslam_get_info", "gslam_set_poses", "gslam_get_poses", "gslam_set_lambda",
+    "gslam_set_log_weights", "gslam_assemble", "gslam_get_system_csr", "gslam_get_system_bsr",
+    "gslam_solve", "gslam_get_dx", "gslam_solve_rhs", "gslam_apply_update", "gslam_gn_step",
+    "gslam_gj_invert", "gslam_last_error",
+    "device_graph_init", "device_graph_update", "device_graph_get_row_ptr",
+    "device_graph_get_col_idx", "device_build_diag_inv", "device_pcg_solve",
+    "device_graph_free", "inv_cuda",
+]
+
+
+def lib():
+    """Load libgslam.so (once).  Raises GSlamError if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise GSlamError(ERR_STATE, "load", f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.gslam_default_opts.argtypes = [vp]
+    L.gslam_default_desc.argtypes = [vp]
+    L.gslam_create.argtypes = [vp, C.POINTER(vp)]
+    L.gslam_destroy.argtypes = [vp]
+    L.gslam_get_info.argtypes = [vp, vp]
+    L.gslam_set_poses.argtypes = [vp, vp]
+    L.gslam_get_poses.argtypes = [vp, vp]
+    L.gslam_set_lambda.argtypes = [vp, C.c_double]
+    L.gslam_set_log_weights.argtypes = [vp, vp]
+    L.gslam_assemble.argtypes = [vp]
+    L.gslam_get_system_csr.argtypes = [vp, vp, vp, vp, vp]
+    L.gslam_get_system_bsr.argtypes = [vp, vp, vp, vp, vp]
+    L.gslam_solve.argtypes = [vp, vp, vp]
+    L.gslam_get_dx.argtypes = [vp, vp]
+    L.gslam_solve_rhs.argtypes = [vp, vp, vp, vp, vp]
+    L.gslam_apply_update.argtypes = [vp, vp]
+    L.gslam_gn_step.argtypes = [vp, vp, vp, vp]
+    L.gslam_gj_invert.argtypes = [vp, vp, C.c_int32, C.c_int32]
+    L.gslam_last_error.restype = C.c_char_p
+    _LIB = L
+    return L
+
+
+def _check(rc: int, where: str, allow=()):
+    if rc != OK and rc not in allow:
+        raise GSlamError(rc, where, lib().gslam_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def read_log(path: str) -> dict:
+    """read_data (src/cpu_pcg_fp64/GSLAM.c:415-493): records u/x/z, any other
+    first token sets delta from its second token; tokens split on ' '."""
+    d = {"delta": 0.0, "u": [], "x": [], "z": []}
+    with open(path, "r") as f:
+        for ln, line in enumerate(f, 1):
+            tok = [t for t in line.split(" ") if t != ""]
+            if not tok:
+                raise ValueError(f"{path}:{ln}: empty record")
+            need = {"u": 3, "x": 4, "z": 4}.get(tok[0], 1)
+            if len(tok) < need + 1:
+                raise ValueError(f"{path}:{ln}: truncated record")
+            if tok[0] == "u":
+                d["u"].append((int(float(tok[1])), float(tok[2]), float(tok[3])))
+            elif tok[0] == "x":
+                d["x"].append((int(float(tok[1])), float(tok[2]), float(tok[3]), float(tok[4])))
+            elif tok[0] == "z":
+                d["z"].append((int(float(tok[1])), int(float(tok[2])), float(tok[3]), float(tok[4])))
+            else:
+                d["delta"] = float(tok[1])
+    u = np.array(d["u"], dtype=np.float64).reshape(-1, 3)
+    x = np.array(d["x"], dtype=np.float64).reshape(-1, 4)
+    z = np.array(d["z"], dtype=np.float64).reshape(-1, 4)
+    return dict(delta=d["delta"], u_step=u[:, 0].astype(np.int32), u_nu=u[:, 1].copy(),
+                u_omega=u[:, 2].copy(), x_step=x[:, 0].astype(np.int32),
+                poses=x[:, 1:].ravel().copy(), z_step=z[:, 0].astype(np.int32),
+                z_lm=z[:, 1].astype(np.int32), z_range=z[:, 2].copy(), z_bearing=z[:, 3].copy(),
+                n_poses=len(x))
+
+
+def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
+              check_every=0) -> _Opts:
+    o = _Opts()
+    lib().gslam_default_opts(C.byref(o))
+    o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
+    o.tol = tol
+    o.max_iter = max_iter
+    o.refine_sweeps = refine_sweeps
+    o.refresh_every = refresh_every
+    o.check_every = check_every
+    return o
+
+
+def _stats(s: _Stats) -> dict:
+    return dict(iters=s.iters, converged=bool(s.converged), reliable_updates=s.reliable_updates,
+                flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms)
+
+
+class Graph:
+    """One libgslam graph handle (device-resident poses, Omega, xi, solver)."""
+
+    def __init__(self, poses, *, motion=False, u_nu=None, u_omega=None, delta=1.0, lam=0.1,
+                 mns=(0.19, 0.001, 0.13, 0.2), obs=None, pair_order=0, snr=(0.14, 0.05),
+                 between=None, anchor=1e6, wrap=0):
+        L = lib()
+        poses = _f64(poses)
+        n = poses.size // 3
+        self._keep = []
+        d = _Desc()
+        L.gslam_default_desc(C.byref(d))
+        d.n_poses = n
+        d.poses = _p(poses)
+        self._keep.append(poses)
+        if motion:
+            u_nu, u_omega = _f64(u_nu, n), _f64(u_omega, n)
+            self._keep += [u_nu, u_omega]
+            d.motion = 1
+            d.u_nu, d.u_omega = _p(u_nu), _p(u_omega)
+        d.delta = delta
+        d.lambda_ = lam
+        for k in range(4):
+            d.mns[k] = mns[k]
+        if obs is not None:
+            zs, zl = _i32(obs["z_step"]), _i32(obs["z_lm"])
+            zr, zb = _f64(obs["z_range"]), _f64(obs["z_bearing"])
+            self._keep += [zs, zl, zr, zb]
+            d.n_obs = zs.size
+            d.z_step, d.z_lm, d.z_range, d.z_bearing = _p(zs), _p(zl), _p(zr), _p(zb)
+        d.pair_order = pair_order
+        d.snr[0], d.snr[1] = snr
+        if between is not None:
+            bi, bj = _i32(between["i"]), _i32(between["j"])
+            bm, bf = _f64(between["meas"], 3 * bi.size), _f64(between["info"], 3 * bi.size)
+            self._keep += [bi, bj, bm, bf]
+            d.n_between = bi.size
+            d.be_i, d.be_j, d.be_meas, d.be_info = _p(bi), _p(bj), _p(bm), _p(bf)
+        d.anchor = anchor
+        d.wrap = wrap
+        h = C.c_void_p()
+        _check(L.gslam_create(C.byref(d), C.byref(h)), "gslam_create")
+        self._keep = []
+        self.h = h
+        self.n = n
+        info = _Info()
+        _check(L.gslam_get_info(self.h, C.byref(info)), "gslam_get_info")
+        self.info = {k: getattr(info, k) for k, _ in _Info._fields_}
+
+    @classmethod
+    def from_log(cls, path: str, lam: float = 0.1, pair_order: int = 0, wrap: int = 0,
+                 data: dict | None = None):
+        g = read_log(path) if data is None else data
+        return cls(g["poses"], motion=True, u_nu=g["u_nu"], u_omega=g["u_omega"],
+                   delta=g["delta"], lam=lam,
+                   obs=dict(z_step=g["z_step"], z_lm=g["z_lm"], z_range=g["z_range"],
+                            z_bearing=g["z_bearing"]),
+                   pair_order=pair_order, wrap=wrap)
+
+    # -------------------------------------------------------------- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gslam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -------------------------------------------------------------- state
+    def set_poses(self, poses):
+        p = _f64(poses, 3 * self.n)
+        _check(lib().gslam_set_poses(self.h, _p(p)), "gslam_set_poses")
+
+    def get_poses(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        _check(lib().gslam_get_poses(self.h, _p(out)), "gslam_get_poses")
+        return out
+
+    def set_lambda(self, lam: float):
+        _check(lib().gslam_set_lambda(self.h, lam), "gslam_set_lambda")
+
+    def set_log_weights(self, w):
+        w = None if w is None else _f64(w, self.info["n_between_edges"])
+        _check(lib().gslam_set_log_weights(self.h, _p(w)), "gslam_set_log_weights")
+
+    # -------------------------------------------------------------- GN pieces
+    def assemble(self):
+        _check(lib().gslam_assemble(self.h), "gslam_assemble")
+
+    def system_csr(self):
+        nnz = self.info["nnz"]
+        rp = np.zeros(3 * self.n + 1, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        va = np.zeros(nnz)
+        xi = np.zeros(3 * self.n)
+        _check(lib().gslam_get_system_csr(self.h, _p(rp), _p(ci), _p(va), _p(xi)),
+               "gslam_get_system_csr")
+        return rp, ci, va, xi
+
+    def system_bsr(self):
+        nb = self.info["n_blocks"]
+        rp = np.zeros(self.n + 1, np.int32)
+        bc = np.zeros(nb, np.int32)
+        bv = np.zeros((nb, 3, 3))
+        xi = np.zeros(3 * self.n)
+        _check(lib().gslam_get_system_bsr(self.h, _p(rp), _p(bc), _p(bv), _p(xi)),
+               "gslam_get_system_bsr")
+        return rp, bc, bv, xi
+
+    def solve(self, solver="pcg64", tol=1e-6, max_iter=500, **kw) -> dict:
+        o = make_opts(solver, tol, max_iter, **kw)
+        s = _Stats()
+        _check(lib().gslam_solve(self.h, C.byref(o), C.byref(s)), "gslam_solve",
+               allow=(ERR_NOT_CONVERGED, ERR_SINGULAR))
+        return _stats(s)
+
+    def dx(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        _check(lib().gslam_get_dx(self.h, _p(out)), "gslam_get_dx")
+        return out
+
+    def solve_rhs(self, rhs, solver="pcg64", tol=1e-10, max_iter=100000, **kw):
+        o = make_opts(solver, tol, max_iter, **kw)
+        b = _f64(rhs, 3 * self.n)
+        x = np.zeros(3 * self.n)
+        s = _Stats()
+        _check(lib().gslam_solve_rhs(self.h, C.byref(o), _p(b), _p(x), C.byref(s)),
+               "gslam_solve_rhs", allow=(ERR_NOT_CONVERGED, ERR_SINGULAR))
+        return x, _stats(s)
+
+    def apply_update(self) -> float:
+        diff = C.c_double()
+        _check(lib().gslam_apply_update(self.h, C.byref(diff)), "gslam_apply_update")
+        return diff.value
+
+    def gn_step(self, solver="pcg64", tol=1e-6, max_iter=500, **kw):
+        o = make_opts(solver, tol, max_iter, **kw)
+        s = _Stats()
+        diff = C.c_double()
+        _check(lib().gslam_gn_step(self.h, C.byref(o), C.byref(s), C.byref(diff)),
+               "gslam_gn_step", allow=(ERR_NOT_CONVERGED, ERR_SINGULAR))
+        return _stats(s), diff.value
+
+
+def gj_invert(a, solver="gj64") -> np.ndarray:
+    """Dense unpivoted Gauss-Jordan inverse on the GPU (host in, host out)."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    out = np.zeros_like(a)
+    _check(lib().gslam_gj_invert(_p(a), _p(out), n, SOLVERS[solver]), "gslam_gj_invert",
+           allow=(ERR_SINGULAR,))
+    return out

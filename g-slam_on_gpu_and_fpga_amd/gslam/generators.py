@@ -1,0 +1,164 @@
+"""Seeded synthetic pose graphs with ground truth (SE(2) between edges).
+
+The reference ships only a landmark-world generator
+(data/generate_citytrees.py:59-204) that writes no ground truth; these two
+pose-graph generators produce odometry + loop-closure "between" edges, the
+dead-reckoned initial guess and the true poses.
+
+grid_city: random walk on a unit Manhattan grid (1 m steps, 90 degree turns
+  with probability p_turn), odometry noise sigma_xy / sigma_th, loop closures
+  between poses closer than lc_radius (true positions) with index gap >
+  lc_gap, each kept with probability lc_keep.
+park: smooth random trajectory (bounded curvature, reflecting off the walls of
+  a side x side square), loop closures within lc_radius with gap > lc_gap,
+  subsampled to about lc_per_pose closures per pose.
+
+Measurements are the true relative poses plus Gaussian noise; information is
+diag(1/sigma^2) per edge (lambda and per-edge weights are applied by the
+solver).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _wrap(a):
+    return (a + np.pi) % (2.0 * np.pi) - np.pi
+
+
+def relative_pose(pi: np.ndarray, pj: np.ndarray) -> np.ndarray:
+    """z = [R(th_i)^T (t_j - t_i), th_j - th_i] for arrays of poses (k, 3)."""
+    c, s = np.cos(pi[:, 2]), np.sin(pi[:, 2])
+    dx, dy = pj[:, 0] - pi[:, 0], pj[:, 1] - pi[:, 1]
+    return np.stack([c * dx + s * dy, -s * dx + c * dy, _wrap(pj[:, 2] - pi[:, 2])], 1)
+
+
+def compose(p: np.ndarray, z: np.ndarray) -> np.ndarray:
+    """p (+) z for single poses."""
+    c, s = np.cos(p[2]), np.sin(p[2])
+    return np.array([p[0] + c * z[0] - s * z[1], p[1] + s * z[0] + c * z[1], p[2] + z[2]])
+
+
+def _dead_reckon(z_odo: np.ndarray, p0: np.ndarray) -> np.ndarray:
+    n = z_odo.shape[0] + 1
+    out = np.zeros((n, 3))
+    out[0] = p0
+    # vectorised composition: headings are a cumulative sum, translations a
+    # cumulative sum of rotated steps
+    th = p0[2] + np.concatenate([[0.0], np.cumsum(z_odo[:, 2])])
+    c, s = np.cos(th[:-1]), np.sin(th[:-1])
+    step = np.stack([c * z_odo[:, 0] - s * z_odo[:, 1], s * z_odo[:, 0] + c * z_odo[:, 1]], 1)
+    out[1:, :2] = p0[:2] + np.cumsum(step, 0)
+    out[:, 2] = th
+    return out
+
+
+def _loop_pairs(xy: np.ndarray, radius: float, gap: int):
+    """All (i, j), i < j - gap, with |xy_i - xy_j| < radius (cell hashing)."""
+    cell = np.floor(xy / radius).astype(np.int64)
+    key = (cell[:, 0] + (1 << 20)) * (1 << 21) + (cell[:, 1] + (1 << 20))
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    uniq, start = np.unique(ks, return_index=True)
+    end = np.append(start[1:], len(ks))
+    lookup = dict(zip(uniq.tolist(), zip(start.tolist(), end.tolist())))
+    I, J = [], []
+    for dcx in (-1, 0, 1):
+        for dcy in (-1, 0, 1):
+            nk = key + dcx * (1 << 21) + dcy
+            for i in range(len(xy)):
+                se = lookup.get(int(nk[i]))
+                if se is None:
+                    continue
+                cand = order[se[0]:se[1]]
+                cand = cand[cand > i + gap]
+                if cand.size:
+                    d = np.hypot(xy[cand, 0] - xy[i, 0], xy[cand, 1] - xy[i, 1])
+                    cand = cand[d < radius]
+                    I.append(np.full(cand.size, i))
+                    J.append(cand)
+    if not I:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64)
+    return np.concatenate(I), np.concatenate(J)
+
+
+def _finish(truth, I, J, rng, sxy, sth, initial_from_odometry=True):
+    n = truth.shape[0]
+    sig = np.array([sxy, sxy, sth])
+    oi = np.arange(n - 1)
+    z_odo = relative_pose(truth[oi], truth[oi + 1]) + rng.normal(size=(n - 1, 3)) * sig
+    z_lc = relative_pose(truth[I], truth[J]) + rng.normal(size=(len(I), 3)) * sig
+    init = _dead_reckon(z_odo, truth[0])
+    init[:, 2] = _wrap(init[:, 2])
+    ei = np.concatenate([oi, I]).astype(np.int32)
+    ej = np.concatenate([oi + 1, J]).astype(np.int32)
+    meas = np.concatenate([z_odo, z_lc])
+    info = np.tile(1.0 / sig ** 2, (len(ei), 1))
+    return dict(n_poses=n, truth=truth, poses=init.ravel().copy(), i=ei, j=ej,
+                meas=meas.ravel().copy(), info=info.ravel().copy(), n_odometry=n - 1,
+                n_loop=len(I))
+
+
+def grid_city(n_poses: int, seed: int = 0, p_turn: float = 0.2, sigma_xy: float = 0.05,
+              sigma_th: float = 0.01, lc_radius: float = 1.5, lc_gap: int = 20,
+              lc_keep: float = 0.3, max_loop_per_pose: int | None = None) -> dict:
+    """Grid-city pose graph (random walk on a unit Manhattan grid)."""
+    rng = np.random.default_rng(seed)
+    turns = rng.random(n_poses - 1) < p_turn
+    sign = np.where(rng.random(n_poses - 1) < 0.5, -1.0, 1.0)
+    dth = np.where(turns, sign * np.pi / 2, 0.0)
+    th = np.concatenate([[0.0], np.cumsum(dth)])
+    step = np.stack([np.round(np.cos(th[1:])), np.round(np.sin(th[1:]))], 1)
+    xy = np.concatenate([[[0.0, 0.0]], np.cumsum(step, 0)])
+    truth = np.concatenate([xy, _wrap(th)[:, None]], 1)
+    I, J = _loop_pairs(xy, lc_radius, lc_gap)
+    keep = rng.random(len(I)) < lc_keep
+    I, J = I[keep], J[keep]
+    if max_loop_per_pose is not None and len(I):
+        perm = rng.permutation(len(I))
+        cnt = np.zeros(n_poses, np.int64)
+        sel = []
+        for k in perm:
+            if cnt[I[k]] < max_loop_per_pose and cnt[J[k]] < max_loop_per_pose:
+                cnt[I[k]] += 1
+                cnt[J[k]] += 1
+                sel.append(k)
+        sel = np.sort(np.array(sel, np.int64))
+        I, J = I[sel], J[sel]
+    return _finish(truth, I, J, rng, sigma_xy, sigma_th)
+
+
+def park(n_poses: int = 7120, seed: int = 0, side: float = 300.0, speed: float = 1.0,
+         sigma_xy: float = 0.1, sigma_th: float = 0.02, lc_radius: float = 3.0,
+         lc_gap: int = 50, lc_per_pose: float = 4.0) -> dict:
+    """Park-like graph: smooth random trajectory inside a side x side square."""
+    rng = np.random.default_rng(seed)
+    th = np.zeros(n_poses)
+    xy = np.zeros((n_poses, 2))
+    xy[0] = side / 2
+    omega = 0.0
+    for k in range(1, n_poses):
+        omega = 0.9 * omega + rng.normal(0.0, 0.03)
+        t = th[k - 1] + omega
+        p = xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)])
+        # reflect off the walls
+        if p[0] < 0 or p[0] > side:
+            t = np.pi - t
+        if p[1] < 0 or p[1] > side:
+            t = -t
+        p = np.clip(xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)]), 0, side)
+        th[k], xy[k] = t, p
+    truth = np.concatenate([xy, _wrap(th)[:, None]], 1)
+    I, J = _loop_pairs(xy, lc_radius, lc_gap)
+    if len(I):
+        want = int(lc_per_pose * n_poses / 2)
+        keep = rng.permutation(len(I))[: min(want, len(I))]
+        keep.sort()
+        I, J = I[keep], J[keep]
+    return _finish(truth, I, J, rng, sigma_xy, sigma_th)
+
+
+def graph_kwargs(g: dict, lam: float, wrap: int = 1) -> dict:
+    """Keyword arguments of gslam.Graph for a generated pose graph."""
+    return dict(lam=lam, between=dict(i=g["i"], j=g["j"], meas=g["meas"], info=g["info"]),
+                wrap=wrap)

@@ -1,0 +1,187 @@
+/*
+ * gslam.h -- C-ABI of libgslam, the MI355X (gfx950) Graph-SLAM back-end.
+ *
+ * One Gauss-Newton (GN) step = edge evaluation + assembly of the 3N x 3N
+ * information matrix Omega and vector xi, a solve Omega dx = xi, and the
+ * pose update.  Everything runs on one GPU through hand-written HIP kernels;
+ * no vendor sparse/BLAS library is used.
+ *
+ * The handle API below replaces the module-static device graph of the
+ * reference GPU PCG (src/gpu_pcg_fp64/device_graph.cuh:23-91) and its dense
+ * GJ entry (src/gpu_gj_fp64/inv.h:9).  The reference's own entry points are
+ * exported unchanged by gslam_compat.h.
+ *
+ * Conventions: plain pointers and sizes, host arrays are caller-owned and
+ * copied in/out, the handle owns every device allocation, every call returns
+ * GSLAM_OK (0) or a negative status and never exits the process.  A handle
+ * is not thread-safe; distinct handles are independent.  Calls are
+ * synchronous at return unless stated otherwise.
+ */
+#ifndef GSLAM_H
+#define GSLAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSLAM_OK             0
+#define GSLAM_ERR_ARG       -1  /* invalid argument / index out of range     */
+#define GSLAM_ERR_HIP       -2  /* HIP runtime error (message: last_error)   */
+#define GSLAM_ERR_NOMEM     -3  /* host or device allocation failed          */
+#define GSLAM_ERR_SINGULAR  -4  /* zero pivot in Gauss-Jordan                */
+#define GSLAM_ERR_STATE     -5  /* call out of order (e.g. solve before assemble) */
+#define GSLAM_ERR_TOO_LARGE -6  /* size beyond a documented limit             */
+#define GSLAM_ERR_NOT_CONVERGED -7 /* solver hit its iteration cap (stats valid) */
+
+typedef struct gslam_graph gslam_graph;
+
+/* Graph description.  Edges are evaluated in this order, which is also the
+ * order of summation into Omega/xi:
+ *   1. landmark-pair observation edges (ObsEdge, src/cpu_pcg_fp64/ObsEdge.c),
+ *      enumerated from the z records per pair_order;
+ *   2. n_poses-1 velocity-model motion edges (i, i+1) when motion != 0
+ *      (MotionEdge, src/cpu_pcg_fp64/MotionEdge.c), using controls u[i+1];
+ *   3. SE(2) between edges (pose-graph odometry / loop closures; no
+ *      reference counterpart).
+ * Items 1+2 with anchor 1e6 and wrap 0 are exactly the reference .log.txt
+ * model (src/cpu_pcg_fp64/GSLAM.c:55-139). */
+typedef struct {
+    int32_t n_poses;
+    const double *poses;        /* 3*n_poses: x, y, theta                    */
+
+    int32_t motion;             /* 1: add the n_poses-1 motion edges         */
+    const double *u_nu;         /* n_poses controls (u records in file order) */
+    const double *u_omega;
+    double delta;               /* time step ("delta" record)               */
+    double lambda;              /* scales motion and between edges          */
+    double mns[4];              /* motion noise {0.19, 0.001, 0.13, 0.2}     */
+
+    int32_t n_obs;              /* z records                                 */
+    const int32_t *z_step;      /* pose index of each observation            */
+    const int32_t *z_lm;        /* landmark id                               */
+    const double *z_range;
+    const double *z_bearing;
+    int32_t pair_order;         /* 0: first-appearance landmark order (CPU
+                                   reference, GSLAM.c:246-276); 1: ascending
+                                   landmark id (GPU reference,
+                                   device_graph.cu:505-551)                  */
+    double snr[2];              /* sensor noise ratios {0.14, 0.05}          */
+
+    int64_t n_between;          /* SE(2) between edges                       */
+    const int32_t *be_i;        /* from pose                                 */
+    const int32_t *be_j;        /* to pose                                   */
+    const double *be_meas;      /* 3 per edge: dx, dy (frame of pose i), dth */
+    const double *be_info;      /* 3 per edge: diagonal information 1/sigma^2 */
+
+    double anchor;              /* added to Omega[d][d], d<3 (1e6)           */
+    int32_t wrap;               /* 0: reference semantics (no angle wrap);
+                                   1: wrap theta residuals and updated poses
+                                   to [-pi, pi)                               */
+} gslam_graph_desc;
+
+typedef struct {
+    int32_t n_poses;
+    int64_t n_edges;            /* obs pairs + motion + between              */
+    int64_t n_obs_edges, n_motion_edges, n_between_edges;
+    int64_t n_blocks;           /* 3x3 blocks of Omega (BSR)                 */
+    int64_t nnz;                /* scalar nonzeros of the reference CSR      */
+    int64_t device_bytes;       /* device memory held by the handle          */
+} gslam_graph_info;
+
+#define GSLAM_SOLVER_PCG_FP64   0  /* block-Jacobi PCG, FP64 throughout      */
+#define GSLAM_SOLVER_PCG_MIXED  1  /* FP32 inner PCG, FP64 true-residual
+                                      reliable updates                      */
+#define GSLAM_SOLVER_GJ_FP64    2  /* dense unpivoted Gauss-Jordan, FP64     */
+#define GSLAM_SOLVER_GJ_MIXED   3  /* blocked GJ, FP32 MFMA trailing updates,
+                                      FP64 iterative refinement             */
+
+typedef struct {
+    int32_t solver;             /* GSLAM_SOLVER_*                            */
+    double tol;                 /* PCG: stop when ||r|| < tol * ||xi||        */
+    int32_t max_iter;           /* PCG iteration cap                         */
+    int32_t refine_sweeps;      /* GJ_MIXED: FP64 refinement sweeps          */
+    int32_t refresh_every;      /* PCG_MIXED: FP64 true-residual refresh and
+                                   convergence test every k iterations (64)  */
+    int32_t check_every;        /* PCG: host polls convergence every k iters */
+} gslam_solve_opts;
+
+typedef struct {
+    int32_t iters;              /* PCG iterations executed                   */
+    int32_t converged;          /* 1 if the tolerance was met                */
+    int32_t reliable_updates;   /* PCG_MIXED: FP64 residual refreshes        */
+    int32_t flags;              /* bit0: p.q == 0 break (MatrixOps.c:357)     */
+    double rel_res;             /* final ||r||/||xi|| of the solver's residual */
+    double true_rel_res;        /* ||xi - Omega dx|| / ||xi||, FP64          */
+    double ms;                  /* device time of the solve (HIP events)     */
+} gslam_solve_stats;
+
+/* Fills opts with the reference defaults (PCG FP64, tol 1e-6, 500 iterations:
+ * src/cpu_pcg_fp64/MatrixOps.c:345-347). */
+void gslam_default_opts(gslam_solve_opts *opts);
+void gslam_default_desc(gslam_graph_desc *desc);
+
+/* Lifecycle: replaces device_graph_init / device_graph_free
+ * (src/gpu_pcg_fp64/device_graph.cuh:23-33, 91).  Builds the 64-bit-safe
+ * sort-based block structure once (no dim x dim marker) and uploads it. */
+int gslam_create(const gslam_graph_desc *desc, gslam_graph **out);
+int gslam_destroy(gslam_graph *g);
+int gslam_get_info(const gslam_graph *g, gslam_graph_info *info);
+
+/* Poses (host <-> device, 3*n_poses doubles). */
+int gslam_set_poses(gslam_graph *g, const double *poses);
+int gslam_get_poses(gslam_graph *g, double *poses);
+int gslam_set_lambda(gslam_graph *g, double lambda);
+/* Per-between-edge information multipliers exp(w_e) given as log-weights
+ * w_e (n_between doubles; NULL resets to 0). */
+int gslam_set_log_weights(gslam_graph *g, const double *log_w);
+
+/* Assembly of Omega and xi at the current poses: replaces
+ * device_graph_update (device_graph.cuh:47-52) minus its host upload.
+ * Deterministic: each entry is summed in edge order starting from the
+ * anchor, the order of build_csr_from_edges (MatrixOps.c:223-272). */
+int gslam_assemble(gslam_graph *g);
+
+/* Download of the assembled system.  CSR in the reference layout
+ * (row_ptr[3N+1], col_idx[nnz], values[nnz]), BSR block rows
+ * (brow_ptr[N+1], bcol[n_blocks], bval[9*n_blocks]); any pointer may be
+ * NULL. */
+int gslam_get_system_csr(gslam_graph *g, int32_t *row_ptr, int32_t *col_idx,
+                         double *values, double *xi);
+int gslam_get_system_bsr(gslam_graph *g, int32_t *brow_ptr, int32_t *bcol,
+                         double *bval, double *xi);
+
+/* Solve Omega dx = xi (replaces device_pcg_solve, device_graph.cuh:83-84,
+ * and the inv_cuda + host matvec of src/gpu_gj_fp64/GSLAM.cu:391-433).
+ * dx stays on the device; GSLAM_ERR_NOT_CONVERGED if the cap was hit (dx
+ * and stats are still valid, as the reference returns silently). */
+int gslam_solve(gslam_graph *g, const gslam_solve_opts *opts,
+                gslam_solve_stats *stats);
+int gslam_get_dx(gslam_graph *g, double *dx);
+/* Solve with a caller right-hand side instead of xi (adjoint solves). */
+int gslam_solve_rhs(gslam_graph *g, const gslam_solve_opts *opts,
+                    const double *rhs, double *x, gslam_solve_stats *stats);
+
+/* poses += dx (src/cpu_pcg_fp64/GSLAM.c:98-102), wrapped when desc.wrap;
+ * diff = ||dx|| / sqrt(3N) (GSLAM.c:104-105). */
+int gslam_apply_update(gslam_graph *g, double *diff_out);
+
+/* assemble + solve + apply_update. */
+int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
+                  gslam_solve_stats *stats, double *diff_out);
+
+/* Dense inverse of a host matrix by unpivoted Gauss-Jordan
+ * (src/cpu_pcg_fp64/MatrixOps.c:29-91; GPU twin src/gpu_gj_fp64/inv.cu).
+ * solver: GSLAM_SOLVER_GJ_FP64 or GSLAM_SOLVER_GJ_MIXED (FP32 MFMA blocked,
+ * no refinement of the inverse itself).  GSLAM_ERR_SINGULAR on a zero pivot
+ * (inv is then all zeros, as in the reference). */
+int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
+
+/* Last error message of the calling thread ("" if none). */
+const char *gslam_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
