@@ -259,8 +259,39 @@ int gslam_assemble(gslam_graph *gh)
     GSLAM_TRY(gslam::launch_assembly(g));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
     g.assembled = true;
+    g.factors_valid = true;
     g.solved = false;
     return GSLAM_OK;
+}
+
+int gslam_set_system_csr(gslam_graph *gh, const double *values, const double *xi)
+{
+    if (!gh || !values || !xi) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    const int64_t nnz = g.h.nnz;
+    double *tmp = nullptr;
+    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&tmp), static_cast<size_t>(nnz > 0 ? nnz : 1) * 8));
+    int rc = g.upload(tmp, values, nnz * 8);
+    if (rc == GSLAM_OK) rc = gslam::launch_csr_to_bsr(g, tmp, g.bval);
+    if (rc == GSLAM_OK) rc = g.upload(g.xi, xi, 3 * static_cast<int64_t>(g.h.n) * 8);
+    (void)hipStreamSynchronize(g.stream);
+    (void)hipFree(tmp);
+    if (rc != GSLAM_OK) return rc;
+    g.assembled = true;
+    g.factors_valid = false;
+    g.solved = false;
+    return GSLAM_OK;
+}
+
+int gslam_time_kernel(gslam_graph *gh, int32_t which, int32_t reps, double *us)
+{
+    if (!gh || !us || reps <= 0) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    if (!g.assembled) {
+        set_error("gslam_time_kernel: call gslam_assemble first");
+        return GSLAM_ERR_STATE;
+    }
+    return gslam::time_kernel(g, which, reps, us);
 }
 
 int gslam_get_system_csr(gslam_graph *gh, int32_t *row_ptr, int32_t *col_idx, double *values,
@@ -325,6 +356,10 @@ static int solve_into(DeviceGraph &g, const gslam_solve_opts &o, const double *b
     std::memset(&s, 0, sizeof(s));
     if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED)
         return gslam::dense_solve(g, o, b, x, s);
+    if (o.solver == GSLAM_SOLVER_PCG_MIXED && !g.factors_valid) {
+        set_error("mixed PCG needs the edge factors of gslam_assemble (system was set from CSR)");
+        return GSLAM_ERR_STATE;
+    }
     GSLAM_TRY(gslam::pcg_solve(g, o, b, x, s));
     if (!s.converged && !(s.flags & 1) && o.max_iter > 0 && s.iters >= o.max_iter)
         return GSLAM_ERR_NOT_CONVERGED;

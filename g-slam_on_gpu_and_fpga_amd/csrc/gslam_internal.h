@@ -53,6 +53,7 @@ struct DeviceGraph {
     int wrap = 0;
     int64_t bytes = 0;
     bool assembled = false, solved = false;
+    bool factors_valid = false;  // frec matches the assembled system
 
     // inputs / poses
     double *poses = nullptr;     // 3N
@@ -137,6 +138,8 @@ int pcg_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b,
 int pose_update(DeviceGraph &g, double *diff_out);
 // y = Omega x (FP64, BSR) on the handle stream.
 int launch_spmv64(DeviceGraph &g, const double *x, double *y);
+// Mean per-launch duration (us) of one solver kernel (GSLAM_KERNEL_*).
+int time_kernel(DeviceGraph &g, int which, int reps, double *us);
 // ||b - Omega x|| / ||b|| in FP64 (device reductions, one host sync).
 int true_rel_residual(DeviceGraph &g, const double *b, const double *x, double *out);
 

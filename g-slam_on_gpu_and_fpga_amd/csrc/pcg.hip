@@ -426,6 +426,52 @@ int run_chunked(DeviceGraph &g, const gslam_solve_opts &o,
     return GSLAM_OK;
 }
 
+int time_mixed_spmv(DeviceGraph &g, int reps, double *us);
+
+int time_kernel(DeviceGraph &g, int which, int reps, double *us)
+{
+    const int32_t n = g.h.n;
+    const unsigned int gs = grid_for(n, kLanesPerRow), gu = grid_for(n, 1);
+    double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
+    double *PQ = g.part + kPartOffPQ;
+    if (which == GSLAM_KERNEL_SPMV_MIXED) return time_mixed_spmv(g, reps, us);
+    auto spmv = [&]() {
+        hipLaunchKernelGGL(pcg_spmv_kernel<kLanesPerRow>, dim3(gs), dim3(kThreads), 0, g.stream,
+                           g.brow_ptr, g.bcol, g.bval, g.z, g.p[0], g.p[1], g.q, n, 0, 1e-30,
+                           PR[0], static_cast<int32_t>(gu), PR[1], static_cast<int32_t>(gu), PQ,
+                           g.status);
+    };
+    auto update = [&]() {
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.p[1], g.q,
+                           g.dinv, g.x, g.r, g.z, n, PQ, static_cast<int32_t>(gs), PR[0],
+                           static_cast<int32_t>(gu), PR[1], g.status);
+    };
+    auto run = [&](int rr) {
+        for (int i = 0; i < rr; ++i) {
+            if (which == GSLAM_KERNEL_SPMV_FP64) spmv();
+            else if (which == GSLAM_KERNEL_UPDATE_FP64) update();
+            else (void)launch_assembly(g);
+        }
+    };
+    if (which < GSLAM_KERNEL_SPMV_FP64 || which > GSLAM_KERNEL_ASSEMBLY) return GSLAM_ERR_ARG;
+    // a consistent solver state: setup, one SpMV (valid p.q partials)
+    hipLaunchKernelGGL(pcg_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
+                       g.xi, n, 1, g.dinv, static_cast<float *>(nullptr), g.x, g.r, g.z, g.p[0],
+                       g.p[1], PR[0], g.status);
+    spmv();
+    run(3);  // warm-up
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    run(reps);
+    GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
+    GSLAM_HIP(hipEventSynchronize(g.ev1));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    *us = 1000.0 * ms / reps;
+    g.solved = false;
+    return GSLAM_OK;
+}
+
 int pcg_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
               gslam_solve_stats &s)
 {

@@ -412,4 +412,38 @@ int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, 
     return GSLAM_OK;
 }
 
+int time_mixed_spmv(DeviceGraph &g, int reps, double *us)
+{
+    if (!g.factors_valid) {
+        set_error("mixed operator timing needs gslam_assemble");
+        return GSLAM_ERR_STATE;
+    }
+    const int32_t n = g.h.n;
+    const unsigned int gs = grid_for(n, kLanesMixed), gu = grid_for(n, 1);
+    double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
+    double *PQ = g.part + kPartOffPQ;
+    const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
+    hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
+                       g.xi, n, 1, g.dinv32, g.x, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
+                       g.status);
+    auto spmv = [&]() {
+        hipLaunchKernelGGL(mixed_spmv_kernel<kLanesMixed>, dim3(gs), dim3(kThreads), 0, g.stream,
+                           g.xptr, g.xlist, g.xoth, frec, g.h.n_obs_e, g.h.n_mot_e,
+                           static_cast<float>(g.anchor), g.z32, g.p32[0], g.p32[1], g.q32, n, 0, 1,
+                           1e-30, PR[0], static_cast<int32_t>(gu), PR[1], static_cast<int32_t>(gu),
+                           PQ, g.status);
+    };
+    for (int i = 0; i < 3; ++i) spmv();
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    for (int i = 0; i < reps; ++i) spmv();
+    GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
+    GSLAM_HIP(hipEventSynchronize(g.ev1));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    *us = 1000.0 * ms / reps;
+    g.solved = false;
+    return GSLAM_OK;
+}
+
 }  // namespace gslam

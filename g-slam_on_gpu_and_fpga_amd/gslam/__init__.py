@@ -23,6 +23,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_STATE, ERR_TOO_LARGE, ERR_NOT_CONVERGED = (
     -1, -2, -3, -4, -5, -6, -7)
 SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3}
+KERNELS = {"spmv64": 0, "update64": 1, "spmv_mixed": 2, "assembly": 3}
 
 
 class GSlamError(RuntimeError):
@@ -70,7 +71,7 @@ EXPORTS = [
     "gslam_get_info", "gslam_set_poses", "gslam_get_poses", "gslam_set_lambda",
     "gslam_set_log_weights", "gslam_assemble", "gslam_get_system_csr", "gslam_get_system_bsr",
     "gslam_solve", "gslam_get_dx", "gslam_solve_rhs", "gslam_apply_update", "gslam_gn_step",
-    "gslam_gj_invert", "gslam_last_error",
+    "gslam_gj_invert", "gslam_last_error", "gslam_set_system_csr", "gslam_time_kernel",
     "device_graph_init", "device_graph_update", "device_graph_get_row_ptr",
     "device_graph_get_col_idx", "device_build_diag_inv", "device_pcg_solve",
     "device_graph_free", "inv_cuda",
@@ -105,6 +106,8 @@ def lib():
     L.gslam_gn_step.argtypes = [vp, vp, vp, vp]
     L.gslam_gj_invert.argtypes = [vp, vp, C.c_int32, C.c_int32]
     L.gslam_last_error.restype = C.c_char_p
+    L.gslam_set_system_csr.argtypes = [vp, vp, vp]
+    L.gslam_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, vp]
     _LIB = L
     return L
 
@@ -295,6 +298,19 @@ class Graph:
         _check(lib().gslam_get_system_bsr(self.h, _p(rp), _p(bc), _p(bv), _p(xi)),
                "gslam_get_system_bsr")
         return rp, bc, bv, xi
+
+    def set_system_csr(self, values, xi):
+        """Load Omega (reference CSR layout on this graph's pattern) and xi."""
+        va = _f64(values, self.info["nnz"])
+        x = _f64(xi, 3 * self.n)
+        _check(lib().gslam_set_system_csr(self.h, _p(va), _p(x)), "gslam_set_system_csr")
+
+    def time_kernel(self, kernel: str, reps: int = 100) -> float:
+        """Mean per-launch microseconds of one solver kernel (HIP events)."""
+        us = C.c_double()
+        _check(lib().gslam_time_kernel(self.h, KERNELS[kernel], reps, C.byref(us)),
+               "gslam_time_kernel")
+        return us.value
 
     def solve(self, solver="pcg64", tol=1e-6, max_iter=500, **kw) -> dict:
         o = make_opts(solver, tol, max_iter, **kw)

@@ -152,6 +152,14 @@ int gslam_get_system_csr(gslam_graph *g, int32_t *row_ptr, int32_t *col_idx,
 int gslam_get_system_bsr(gslam_graph *g, int32_t *brow_ptr, int32_t *bcol,
                          double *bval, double *xi);
 
+/* Replaces the assembled system by caller values in the reference CSR layout
+ * (values[nnz] on the pattern gslam_get_system_csr reports, xi[3N]), e.g. a
+ * system assembled by the reference CPU code (build_csr_from_edges,
+ * src/cpu_pcg_fp64/MatrixOps.c:173-273).  The mixed solver needs the edge
+ * factors gslam_assemble writes and returns GSLAM_ERR_STATE until the next
+ * gslam_assemble. */
+int gslam_set_system_csr(gslam_graph *g, const double *values, const double *xi);
+
 /* Solve Omega dx = xi (replaces device_pcg_solve, device_graph.cuh:83-84,
  * and the inv_cuda + host matvec of src/gpu_gj_fp64/GSLAM.cu:391-433).
  * dx stays on the device; GSLAM_ERR_NOT_CONVERGED if the cap was hit (dx
@@ -177,6 +185,15 @@ int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
  * no refinement of the inverse itself).  GSLAM_ERR_SINGULAR on a zero pivot
  * (inv is then all zeros, as in the reference). */
 int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
+
+/* Kernel timing on the current system: runs one kernel `reps` times back to
+ * back on the handle's stream and returns the mean duration per launch in
+ * microseconds, measured with HIP events on that stream. */
+#define GSLAM_KERNEL_SPMV_FP64   0  /* FP64 PCG: BSR SpMV + p.q partials       */
+#define GSLAM_KERNEL_UPDATE_FP64 1  /* FP64 PCG: x, r, z update + partials      */
+#define GSLAM_KERNEL_SPMV_MIXED  2  /* mixed PCG: factorized FP32 operator      */
+#define GSLAM_KERNEL_ASSEMBLY    3  /* edge evaluation + Omega/xi gathers       */
+int gslam_time_kernel(gslam_graph *g, int32_t which, int32_t reps, double *us_per_launch);
 
 /* Last error message of the calling thread ("" if none). */
 const char *gslam_last_error(void);

@@ -1,0 +1,84 @@
+"""CPU-side checks of the C-ABI boundary: libgslam.so loads, exports every
+function include/*.h declares, and fails cleanly (an error code, not a crash or
+a CPU fallback) when no GPU is present.  No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+import gslam
+
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("gslam.h", "gslam_compat.h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|void|char)\s*\**\s*(\w+)\s*\(", text, re.M):
+            names.append(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25, names
+    L = gslam.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the binding's list covers the same set
+    assert set(names) == set(gslam.EXPORTS), set(names) ^ set(gslam.EXPORTS)
+
+
+def test_defaults_reproduce_reference_constants():
+    o = gslam.make_opts()
+    assert (o.solver, o.tol, o.max_iter) == (0, 1e-6, 500)  # MatrixOps.c:345-347
+    d = gslam._Desc()
+    gslam.lib().gslam_default_desc(C.byref(d))
+    assert list(d.mns) == [0.19, 0.001, 0.13, 0.2]           # GSLAM.c:66
+    assert list(d.snr) == [0.14, 0.05]                        # GSLAM.c:301
+    assert d.anchor == 1e6                                    # MatrixOps.c:227
+    assert d.wrap == 0 and d.pair_order == 0
+
+
+def _gpu_present():
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_present(), reason="checks the no-GPU error path")
+def test_create_without_gpu_is_an_error_not_a_fallback():
+    with pytest.raises(gslam.GSlamError) as ei:
+        gslam.Graph([0.0, 0.0, 0.0, 1.0, 0.0, 0.0],
+                    between=dict(i=[0], j=[1], meas=[1.0, 0, 0], info=[1.0, 1, 1]))
+    assert ei.value.code == gslam.ERR_HIP
+
+
+def test_argument_validation_before_any_device_call():
+    """Invalid descriptions are rejected by the host-side structure build."""
+    with pytest.raises(gslam.GSlamError) as ei:
+        gslam.Graph([0.0, 0.0, 0.0], between=dict(i=[0], j=[5], meas=[1.0, 0, 0],
+                                                  info=[1.0, 1, 1]))
+    assert ei.value.code == gslam.ERR_ARG
+    with pytest.raises(gslam.GSlamError) as ei:
+        gslam.Graph([0.0, 0.0, 0.0, 1, 0, 0], obs=dict(z_step=[0, 7], z_lm=[1, 1],
+                                                       z_range=[1.0, 1.0], z_bearing=[0.0, 0.0]))
+    assert ei.value.code == gslam.ERR_ARG
+
+
+def test_read_log_matches_oracle_reader(oracle):
+    from conftest import data_path
+    import numpy as np
+    for f in ("cityTrees100.log.txt", "victoria-park-100.log.txt", "cityTrees800.txt"):
+        a = gslam.read_log(data_path(f))
+        b = oracle.read_log(data_path(f))
+        for k in ("u_nu", "u_omega", "poses", "z_range", "z_bearing", "z_step", "z_lm"):
+            assert np.array_equal(a[k], b[k]), (f, k)
+        assert a["delta"] == b["delta"]
