@@ -93,6 +93,8 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.csr_row_ptr, dim + 1));
     GSLAM_TRY(g.alloc(&g.csr_col_idx, h.nnz));
     GSLAM_TRY(g.alloc(&g.csr_pos, 9 * B));
+    g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
+    GSLAM_TRY(g.alloc(&g.tile_row, h.tile_row.size()));
     GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
     GSLAM_TRY(g.alloc(&g.bval32, 9 * B));
@@ -138,6 +140,7 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.upload(g.csr_row_ptr, h.csr_row_ptr.data(), (dim + 1) * 4));
     GSLAM_TRY(g.upload(g.csr_col_idx, h.csr_col_idx.data(), h.nnz * 4));
     GSLAM_TRY(g.upload(g.csr_pos, h.csr_pos.data(), 9 * B * 4));
+    GSLAM_TRY(g.upload(g.tile_row, h.tile_row.data(), h.tile_row.size() * 4));
     GSLAM_TRY(g.upload(g.xoth, h.xoth.data(), 2 * E * 4));
     GSLAM_TRY(g.upload(g.xpos, h.xpos.data(), 2 * E * 4));
     return GSLAM_OK;

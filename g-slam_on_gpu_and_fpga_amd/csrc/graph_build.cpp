@@ -159,6 +159,21 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
     }
     g.n_blocks = static_cast<int64_t>(g.bcol.size());
 
+    // SpMV tiles: greedy packing of whole block rows
+    g.tile_row.clear();
+    g.tile_row.push_back(0);
+    {
+        int32_t r0 = 0;
+        while (r0 < N) {
+            int32_t r1 = r0 + 1;
+            while (r1 < N && r1 - r0 < kTileRows &&
+                   g.brow_ptr[r1 + 1] - g.brow_ptr[r0] <= kTileBlocks)
+                ++r1;
+            g.tile_row.push_back(r1);
+            r0 = r1;
+        }
+    }
+
     auto slot = [&](int32_t row, int32_t col) -> int32_t {
         auto first = g.bcol.begin() + g.brow_ptr[row];
         auto last = g.bcol.begin() + g.brow_ptr[row + 1];

@@ -17,6 +17,10 @@ enum { Q_UL = 0, Q_UR = 1, Q_BL = 2, Q_BR = 3 };
 // Per-edge staged values: 4 blocks x 9 + 2 x 3 xi = 42 doubles, padded to 48
 // (384 bytes = three 128-byte cache lines per edge record).
 constexpr int kStage = 48;
+// SpMV tile limits: 256 blocks (2304 values, one per thread of a 256-thread
+// workgroup in 9 passes) and 85 block rows (3 outputs per row <= 256 threads).
+constexpr int kTileBlocks = 256;
+constexpr int kTileRows = 85;
 
 struct HostGraph {
     int32_t n = 0;                 // poses
@@ -40,6 +44,11 @@ struct HostGraph {
     // Per incidence entry k of xlist: the other pose of the edge; and the
     // inverse map xpos[2e+side] = k.
     std::vector<int32_t> xoth, xpos;
+
+    // SpMV tiles: tile t covers block rows [tile_row[t], tile_row[t+1]),
+    // whole rows, at most kTileBlocks blocks and kTileRows rows unless a
+    // single row is longer (then it is a tile of its own).
+    std::vector<int32_t> tile_row;
 
     // Reference CSR (src/cpu_pcg_fp64/MatrixOps.c:173-221 pattern): row_ptr,
     // col_idx and, per BSR value slot (9*b+k), its CSR position or -1.

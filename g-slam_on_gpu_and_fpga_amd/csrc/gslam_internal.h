@@ -66,6 +66,8 @@ struct DeviceGraph {
     int32_t *brow_ptr = nullptr, *bcol = nullptr, *bdiag = nullptr;
     int32_t *cptr = nullptr, *clist = nullptr, *xptr = nullptr, *xlist = nullptr;
     int32_t *csr_row_ptr = nullptr, *csr_col_idx = nullptr, *csr_pos = nullptr;
+    int32_t *tile_row = nullptr;  // SpMV tiles of whole block rows
+    int32_t n_tiles = 0;
 
     // assembled system
     double *stage = nullptr;     // kStage per edge
@@ -76,7 +78,7 @@ struct DeviceGraph {
     // solver work space
     double *x = nullptr, *r = nullptr, *z = nullptr, *q = nullptr;
     double *p[2] = {nullptr, nullptr};
-    double *dinv = nullptr;      // 9 per pose, FP64
+    double *dinv = nullptr;      // FP64 D^-1, column-major per entry: [c][3i+a]
     float *dinv32 = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
     double *r64 = nullptr;       // mixed: FP64 true residual scratch

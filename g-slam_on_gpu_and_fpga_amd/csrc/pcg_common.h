@@ -18,7 +18,13 @@ constexpr int kPartOffPR0 = 0 * kPartStride;    // r.r, r.z of even iterations (
 constexpr int kPartOffPR1 = 2 * kPartStride;    // r.r, r.z of odd iterations (2 arrays)
 constexpr int kPartOffPQ = 4 * kPartStride;     // p.q (1 array)
 constexpr int kPartOffResid = 6 * kPartStride;  // residual / update norms (2 arrays)
-constexpr int kPartTotal = 8 * kPartStride;
+// The SpMV runs one workgroup per row tile (up to kSpmvMaxGrid, grid-stride
+// beyond), so its p.q partials get an array of their own.
+constexpr int kSpmvMaxGrid = 4096;
+constexpr int kPartOffPQT = 8 * kPartStride;    // p.q of the tiled SpMV
+constexpr int kPartTotal = 8 * kPartStride + kSpmvMaxGrid;
+
+constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 
 inline unsigned int grid_for(int64_t rows, int lanes)
 {
@@ -28,12 +34,20 @@ inline unsigned int grid_for(int64_t rows, int lanes)
     return static_cast<unsigned int>(g);
 }
 
+inline unsigned int grid_entries(int64_t n_poses)
+{
+    int64_t g = (3 * n_poses + kUpdThreads - 1) / kUpdThreads;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return static_cast<unsigned int>(g);
+}
+
 // Block sums of K values, written by thread 0 to out[k * stride + blockIdx.x].
-template <int K>
+template <int K, int NT = kThreads>
 __device__ inline void write_partials(double (&v)[K], double *out, int stride)
 {
-    __shared__ double lds[K][kThreads / kWave];
-    block_sum<K, kThreads>(v, lds);
+    __shared__ double lds[K][NT / kWave];
+    block_sum<K, NT>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) out[k * stride + blockIdx.x] = v[k];
@@ -42,18 +56,18 @@ __device__ inline void write_partials(double (&v)[K], double *out, int stride)
 
 // Sum of np partials per quantity in a fixed order; every thread of every
 // workgroup obtains the same bits.
-template <int K>
+template <int K, int NT = kThreads>
 __device__ inline void sum_partials(const double *part, int np, int stride, double (&out)[K])
 {
-    __shared__ double lds[K][kThreads / kWave];
+    __shared__ double lds[K][NT / kWave];
     __shared__ double res[K];
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         v[k] = 0.0;
-        for (int i = threadIdx.x; i < np; i += kThreads) v[k] += part[k * stride + i];
+        for (int i = threadIdx.x; i < np; i += NT) v[k] += part[k * stride + i];
     }
-    block_sum<K, kThreads>(v, lds);
+    block_sum<K, NT>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) res[k] = v[k];
