@@ -13,8 +13,10 @@
 // (src/gpu_pcg_fp64/GSLAM.cu:165-171) the diff is always ||dx||/sqrt(dim).
 //
 // Extra knobs (environment, defaults reproduce the reference):
-//   GSLAM_SOLVER   pcg64 | mixed | gj64        (pcg64)
+//   GSLAM_SOLVER   pcg64 | mixed | gj64 | gjmixed  (pcg64)
 //   GSLAM_TOL      PCG relative tolerance      (1e-6, MatrixOps.c:345)
+//   GSLAM_REFINE   gjmixed refinement: ir | cg (ir)
+//   GSLAM_SWEEPS   gjmixed refinement iterations (3)
 //   GSLAM_MAX_ITER PCG iteration cap           (500,  MatrixOps.c:347)
 //   GSLAM_WRAP     1: wrap theta residuals and poses (0)
 //   GSLAM_ORDER    landmark pair order 0 (CPU reference) | 1 (GPU reference)
@@ -160,6 +162,10 @@ int main(int argc, char *argv[])
     const char *solver = std::getenv("GSLAM_SOLVER");
     if (solver && !std::strcmp(solver, "mixed")) opts.solver = GSLAM_SOLVER_PCG_MIXED;
     if (solver && !std::strcmp(solver, "gj64")) opts.solver = GSLAM_SOLVER_GJ_FP64;
+    if (solver && !std::strcmp(solver, "gjmixed")) opts.solver = GSLAM_SOLVER_GJ_MIXED;
+    const char *refine = std::getenv("GSLAM_REFINE");
+    if (refine && !std::strcmp(refine, "cg")) opts.refine_mode = GSLAM_REFINE_CG;
+    opts.refine_sweeps = env_int("GSLAM_SWEEPS", opts.refine_sweeps);
     opts.tol = env_double("GSLAM_TOL", opts.tol);
     opts.max_iter = env_int("GSLAM_MAX_ITER", opts.max_iter);
     const bool stats_json = env_int("GSLAM_STATS", 0) != 0;

@@ -28,6 +28,8 @@ DeviceGraph::~DeviceGraph()
     if (scal_host) (void)hipHostFree(scal_host);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev2) (void)hipEventDestroy(ev2);
+    dense.release();
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -68,6 +70,7 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     GSLAM_HIP(hipEventCreate(&g.ev0));
     GSLAM_HIP(hipEventCreate(&g.ev1));
+    GSLAM_HIP(hipEventCreate(&g.ev2));
     GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[0], hipEventDisableTiming));
     GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[1], hipEventDisableTiming));
     GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.status_host), sizeof(PcgStatus)));
@@ -162,6 +165,7 @@ void gslam_default_opts(gslam_solve_opts *o)
     o->refine_sweeps = 3;
     o->refresh_every = 64;
     o->check_every = 0;
+    o->refine_mode = GSLAM_REFINE_IR;
 }
 
 void gslam_default_desc(gslam_graph_desc *d)
@@ -217,7 +221,7 @@ int gslam_get_info(const gslam_graph *gh, gslam_graph_info *info)
     info->n_between_edges = h.n_bet_e;
     info->n_blocks = h.n_blocks;
     info->nnz = h.nnz;
-    info->device_bytes = gh->g.bytes;
+    info->device_bytes = gh->g.bytes + gh->g.dense.bytes;
     return GSLAM_OK;
 }
 
@@ -290,6 +294,13 @@ int gslam_time_kernel(gslam_graph *gh, int32_t which, int32_t reps, double *us)
 {
     if (!gh || !us || reps <= 0) return GSLAM_ERR_ARG;
     DeviceGraph &g = gh->g;
+    if (which == GSLAM_KERNEL_GJ_TRAILING) {
+        if (!g.dense.a) {
+            set_error("gslam_time_kernel: the GJ trailing update needs a GJ_MIXED solve first");
+            return GSLAM_ERR_STATE;
+        }
+        return gslam::bgj_time_trailing(g.stream, g.dense, reps, g.ev0, g.ev1, us);
+    }
     if (!g.assembled) {
         set_error("gslam_time_kernel: call gslam_assemble first");
         return GSLAM_ERR_STATE;
@@ -348,6 +359,11 @@ static int check_opts(const gslam_solve_opts *o)
     }
     if (!(o->tol >= 0.0) || o->max_iter < 0) {
         set_error("tol must be >= 0 and max_iter >= 0");
+        return GSLAM_ERR_ARG;
+    }
+    if (o->solver == GSLAM_SOLVER_GJ_MIXED &&
+        (o->refine_sweeps < 0 || (o->refine_mode != GSLAM_REFINE_IR && o->refine_mode != GSLAM_REFINE_CG))) {
+        set_error("GJ_MIXED: refine_sweeps must be >= 0 and refine_mode IR or CG");
         return GSLAM_ERR_ARG;
     }
     return GSLAM_OK;

@@ -138,13 +138,16 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
                 gslam_solve_stats &st)
 {
     const int64_t n = 3 * static_cast<int64_t>(g.h.n);
+    if (o.solver == GSLAM_SOLVER_GJ_MIXED) {
+        if (n > 131072) {
+            set_error("GJ_MIXED limited to 3N <= 131072 (68.7 GB FP32 matrix)");
+            return GSLAM_ERR_TOO_LARGE;
+        }
+        return dense_solve_mixed(g, o, b, x, st);
+    }
     if (n > 60000) {
         set_error("dense solve limited to 3N <= 60000 (28.8 GB FP64 matrix)");
         return GSLAM_ERR_TOO_LARGE;
-    }
-    if (o.solver != GSLAM_SOLVER_GJ_FP64) {
-        set_error("GJ_MIXED is not available in this build");
-        return GSLAM_ERR_ARG;
     }
     double *m = nullptr, *work = nullptr;
     int32_t *flag = nullptr;
@@ -176,6 +179,7 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
     (void)hipFree(flag);
     if (rc != GSLAM_OK) return rc;
     st.ms = ms;
+    st.ms_factor = ms;
     st.iters = 0;
     st.converged = singular ? 0 : 1;
     GSLAM_TRY(true_rel_residual(g, b, x, &st.true_rel_res));
@@ -193,8 +197,9 @@ extern "C" int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t 
 {
     using namespace gslam;
     if (!a || !inv || n <= 0) return GSLAM_ERR_ARG;
+    if (solver == GSLAM_SOLVER_GJ_MIXED) return bgj_invert_host(a, inv, n);
     if (solver != GSLAM_SOLVER_GJ_FP64) {
-        set_error("GJ_MIXED is not available in this build");
+        set_error("gslam_gj_invert: solver must be GJ_FP64 or GJ_MIXED");
         return GSLAM_ERR_ARG;
     }
     const int64_t nn = static_cast<int64_t>(n) * n;

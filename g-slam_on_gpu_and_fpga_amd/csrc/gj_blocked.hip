@@ -1,0 +1,772 @@
+// Blocked Gauss-Jordan inverse in FP32 with MFMA trailing updates, and the
+// GJ_MIXED dense solve: FP32 inverse of the Jacobi-scaled Omega, then FP64
+// refinement on the true residual xi - Omega dx.
+//
+// The reference inverts Omega with an unblocked, unpivoted Gauss-Jordan
+// (inv, src/cpu_pcg_fp64/MatrixOps.c:29-91; inv_float,
+// src/cpu_gj_mixed/MatrixOps.c:174-212; inv_cuda, src/gpu_gj_fp64/inv.cu:
+// 221-354): one pass over the whole matrix per pivot.  Here the pivots are
+// taken kNb = 128 at a time (still unpivoted, in place):
+//
+//   P    = A_KK^-1                      FP64, one workgroup (pivot kernel)
+//   R_J  = P A_KJ            (J != K)   FP64 MFMA, stored FP32 (row panel)
+//   CT   = -A_IK^T           (I != K)   transposed copy of the column panel
+//   A_IJ = A_IJ + CT^T R_J   (I, J != K)  FP32 MFMA 32x32x2, FP32 accumulate
+//   A_IK = CT^T P            (I != K)   same kernel, R_K = P
+//   A_KJ = R_J, A_KK = P
+//
+// which is 2 n^3 flops in total, almost all in the trailing update.  The
+// matrix is padded to a multiple of kNb with an identity block (its inverse
+// is the identity, and exact zeros stay exact, so the padding never mixes
+// into the result).
+//
+// Precision: Omega is scaled to S = D^-1/2 Omega D^-1/2 (unit diagonal) in
+// FP64 before rounding to FP32; the pivot blocks are inverted and the pivot
+// rows scaled in FP64; the trailing GEMMs run in FP32.  The FP32 inverse M
+// then preconditions FP64 refinement on Omega itself:
+//   GSLAM_REFINE_IR: x += D^-1/2 M D^-1/2 (xi - Omega x), refine_sweeps times;
+//   GSLAM_REFINE_CG: conjugate gradients preconditioned by the same operator,
+//     flexible (Polak-Ribiere) beta because M is not exactly symmetric,
+//     residual recomputed in FP64 every iteration.
+// Both stop early when ||xi - Omega x|| < tol ||xi||.
+#include "device_common.h"
+#include "pcg_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace gslam {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNb = 128;          // panel width, trailing tile edge and GEMM depth
+constexpr int kPivThreads = 1024; // pivot kernel: 128 rows x 8 column phases
+constexpr int kXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// ---------------------------------------------------------------- loading
+
+// dsc[i] = 1/sqrt(|d_i|) (1 if d_i == 0) of the diagonal of Omega (BSR) for
+// i < n, 1 on the padding; u (padding part) = 0.
+__global__ void __launch_bounds__(kThreads)
+bgj_scale_bsr_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ bval,
+                     int64_t n, int64_t np, double *__restrict__ dsc)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= np) return;
+    double s = 1.0;
+    if (i < n) {
+        const int64_t row = i / 3;
+        const int a = static_cast<int>(i - 3 * row);
+        const double d = fabs(bval[9 * static_cast<int64_t>(bdiag[row]) + 4 * a]);
+        if (d > 0.0) s = 1.0 / sqrt(d);
+    }
+    dsc[i] = s;
+}
+
+__global__ void __launch_bounds__(kThreads)
+bgj_scale_dense_kernel(const double *__restrict__ m, int64_t n, int64_t np,
+                       double *__restrict__ dsc)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= np) return;
+    double s = 1.0;
+    if (i < n) {
+        const double d = fabs(m[i * n + i]);
+        if (d > 0.0) s = 1.0 / sqrt(d);
+    }
+    dsc[i] = s;
+}
+
+// a = identity on the padding, zero elsewhere (np x np FP32, 4 per thread).
+__global__ void __launch_bounds__(kThreads)
+bgj_clear_kernel(float *__restrict__ a, int64_t n, int64_t np)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t e = 4 * t;
+    if (e >= np * np) return;
+    const int64_t i = e / np, j = e - i * np;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i >= n && i >= j && i < j + 4) {
+        const int d = static_cast<int>(i - j);
+        if (d == 0) v.x = 1.f; else if (d == 1) v.y = 1.f; else if (d == 2) v.z = 1.f; else v.w = 1.f;
+    }
+    *reinterpret_cast<float4 *>(a + e) = v;
+}
+
+// Scaled BSR blocks into the dense FP32 matrix: one thread per block entry.
+__global__ void __launch_bounds__(kThreads)
+bgj_scatter_bsr_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                       const double *__restrict__ bval, int64_t n_blocks, int32_t n_rows,
+                       const double *__restrict__ dsc, int64_t np, float *__restrict__ a)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= 9 * n_blocks) return;
+    const int64_t b = t / 9;
+    const int k = static_cast<int>(t - 9 * b);
+    int32_t lo = 0, hi = n_rows - 1;  // block row of b
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (brow_ptr[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const int64_t i = 3 * static_cast<int64_t>(lo) + k / 3;
+    const int64_t j = 3 * static_cast<int64_t>(bcol[b]) + k % 3;
+    a[i * np + j] = static_cast<float>(bval[t] * dsc[i] * dsc[j]);
+}
+
+// Scaled dense FP64 matrix (n x n) into the padded FP32 matrix.
+__global__ void __launch_bounds__(kThreads)
+bgj_load_dense_kernel(const double *__restrict__ m, int64_t n, int64_t np,
+                      const double *__restrict__ dsc, float *__restrict__ a)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= np * np) return;
+    const int64_t i = t / np, j = t - i * np;
+    float v;
+    if (i < n && j < n) v = static_cast<float>(m[i * n + j] * dsc[i] * dsc[j]);
+    else v = (i == j) ? 1.f : 0.f;
+    a[t] = v;
+}
+
+// inv = D^-1/2 M D^-1/2 (FP64 out, n x n); zeros if the inversion failed.
+__global__ void __launch_bounds__(kThreads)
+bgj_store_dense_kernel(const float *__restrict__ a, int64_t n, int64_t np,
+                       const double *__restrict__ dsc, const int32_t *__restrict__ singular,
+                       double *__restrict__ m)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t i = t / n, j = t - i * n;
+    m[t] = *singular ? 0.0 : static_cast<double>(a[i * np + j]) * dsc[i] * dsc[j];
+}
+
+// ---------------------------------------------------------------- block step
+
+// P = A_KK^-1 in FP64 by in-place Gauss-Jordan.  Thread t owns row t/8 and
+// columns (t%8) + 8q, q < 16, in registers; per pivot the owners publish the
+// pivot row and column to LDS.  Writes A_KK = P and R_K = P (FP32) and
+// PT = P^T (FP64) for the row-panel kernel.  A zero pivot sets *singular
+// (the whole inverse is then reported as zero, MatrixOps.c:60-68).
+__global__ void __launch_bounds__(kPivThreads)
+bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
+                 float *__restrict__ rp, int32_t *__restrict__ singular)
+{
+    __shared__ double rowbuf[kNb], colbuf[kNb];
+    const int t = threadIdx.x;
+    const int r = t >> 3, c0 = t & 7;
+    const int64_t k0 = static_cast<int64_t>(kp) * kNb;
+    double m[16];
+    const float *src = a + (k0 + r) * np + k0 + c0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) m[q] = static_cast<double>(src[8 * q]);
+    bool zero = false;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        for (int s = 0; s < 8; ++s) {
+            const int p = 8 * q + s;
+            if (r == p) {
+#pragma unroll
+                for (int qq = 0; qq < 16; ++qq) rowbuf[c0 + 8 * qq] = m[qq];
+            }
+            if (c0 == s) colbuf[r] = m[q];
+            __syncthreads();
+            double piv = rowbuf[p];
+            if (piv == 0.0) {
+                zero = true;
+                piv = 1.0;
+            }
+            const double ip = 1.0 / piv;
+            if (r == p) {
+#pragma unroll
+                for (int qq = 0; qq < 16; ++qq) m[qq] = (c0 + 8 * qq == p) ? ip : m[qq] * ip;
+            } else {
+                const double f = colbuf[r] * ip;
+#pragma unroll
+                for (int qq = 0; qq < 16; ++qq)
+                    m[qq] = (c0 + 8 * qq == p) ? -f : m[qq] - f * rowbuf[c0 + 8 * qq];
+            }
+            __syncthreads();
+        }
+    }
+    if (zero && c0 == 0) *singular = 1;
+    float *dst = a + (k0 + r) * np + k0 + c0;
+    float *rdst = rp + static_cast<int64_t>(r) * np + k0 + c0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int c = c0 + 8 * q;
+        dst[8 * q] = static_cast<float>(m[q]);
+        rdst[8 * q] = static_cast<float>(m[q]);
+        pt[c * kNb + r] = m[q];
+    }
+}
+
+// Row panel and column panel of block step kp.
+//  blockIdx.x < np/32: R[:, j0:j0+32] = P A[K, j0:j0+32] with FP64 MFMA
+//    16x16x4 (wave w: rows 32w..32w+31 of R, 2 x 2 blocks), written to R
+//    and to row panel K of A (FP32); column tiles inside panel K skip.
+//  otherwise: CT[:, i0:i0+64] = -A[i0:i0+64, K]^T through an LDS transpose,
+//    row tiles inside panel K skip.
+__global__ void __launch_bounds__(kThreads)
+bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__restrict__ pt,
+                 float *__restrict__ rp, float *__restrict__ ct)
+{
+    const int64_t k0 = static_cast<int64_t>(kp) * kNb;
+    const int32_t n_col_tiles = static_cast<int32_t>(np / 32);
+    if (static_cast<int32_t>(blockIdx.x) < n_col_tiles) {
+        const int64_t j0 = 32 * static_cast<int64_t>(blockIdx.x);
+        if (j0 / kNb == kp) return;
+        const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+        const int l16 = lane & 15, h = lane >> 4;
+        f64x4 acc[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const double *pa = pt + static_cast<int64_t>(h) * kNb + 32 * wave + l16;
+        const float *pb = a + (k0 + h) * np + j0 + l16;
+#pragma unroll 4
+        for (int s = 0; s < kNb / 4; ++s) {
+            const double a0 = pa[4 * s * kNb], a1 = pa[4 * s * kNb + 16];
+            const double b0 = static_cast<double>(pb[4 * s * np]);
+            const double b1 = static_cast<double>(pb[4 * s * np + 16]);
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();  // every wave has read rows K of A before any is overwritten
+        // 16x16x4 FP64 accumulator layout (measured on gfx950): register e of
+        // lane l holds D[l/16 + 4e][l%16]
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t row = 32 * wave + 16 * x + h + 4 * e;
+                    const int64_t col = j0 + 16 * y + l16;
+                    const float v = static_cast<float>(acc[x][y][e]);
+                    rp[row * np + col] = v;
+                    a[(k0 + row) * np + col] = v;
+                }
+        return;
+    }
+    __shared__ float tile[kNb][65];
+    const int64_t i0 = 64 * static_cast<int64_t>(blockIdx.x - n_col_tiles);
+    if (i0 >= np || i0 / kNb == kp) return;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < 64 * kNb; e += kThreads) {
+        const int i = e / kNb, k = e % kNb;
+        tile[k][i] = a[(i0 + i) * np + k0 + k];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int e = threadIdx.x; e < 64 * kNb; e += kThreads) {
+        const int k = e / 64, i = e % 64;
+        ct[static_cast<int64_t>(k) * np + i0 + i] = -tile[k][i];
+    }
+}
+
+// Trailing update of block step kp: every 128 x 128 tile (ti, tj), ti != kp,
+//   A_tile = (tj == kp ? 0 : A_tile) + CT[:, ti]^T R[:, tj]
+// with MFMA 32x32x2 FP32 (FP32 accumulate).  Four waves per workgroup, a 64
+// x 64 quadrant each (2 x 2 MFMA blocks, 64 accumulators per lane); the
+// operands come straight from L2 in MFMA lane layout (CT and R are k-major,
+// so lanes 0-31 and 32-63 each read 128 contiguous bytes) and the old tile
+// is loaded into the accumulators before the k loop.  Workgroups are dealt
+// to the 8 XCDs round-robin; tile t = xcd * per_xcd + slot keeps the tiles of
+// one block row (one CT slab) on one XCD's L2.
+__global__ void __launch_bounds__(kThreads)
+bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
+                    const float *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
+                    int32_t per_xcd)
+{
+    const int32_t w = blockIdx.x;
+    const int32_t t = (w % kXcd) * per_xcd + w / kXcd;
+    if (t >= (nt - 1) * nt) return;
+    int32_t ti = t / nt;
+    const int32_t tj = t - ti * nt;
+    if (ti >= kp) ++ti;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int64_t i0 = static_cast<int64_t>(ti) * kNb + 64 * (wave >> 1);
+    const int64_t j0 = static_cast<int64_t>(tj) * kNb + 64 * (wave & 1);
+    f32x16 acc[2][2];
+    float *cbase = a + (i0 + 4 * h) * np + j0 + l32;
+    if (tj != kp) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    acc[x][y][e] = cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y];
+    } else {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[x][y][e] = 0.f;
+    }
+    const float *pa = ct + static_cast<int64_t>(h) * np + i0 + l32;
+    const float *pb = rp + static_cast<int64_t>(h) * np + j0 + l32;
+#pragma unroll 8
+    for (int s = 0; s < kNb / 2; ++s) {
+        const int64_t o = 2 * static_cast<int64_t>(s) * np;
+        const float a0 = pa[o], a1 = pa[o + 32];
+        const float b0 = pb[o], b1 = pb[o + 32];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y] = acc[x][y][e];
+}
+
+// ---------------------------------------------------------------- refinement
+
+// MODE 0: out = Omega v, partial v.out.
+// MODE 1: out = b - Omega v (v == NULL: v = 0), u = dsc * out, partial out.out
+//         and (first call) b.b.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+bgj_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                const double *__restrict__ bval, const double *__restrict__ v,
+                const double *__restrict__ b, const double *__restrict__ dsc,
+                double *__restrict__ out, double *__restrict__ u, int32_t n,
+                double *__restrict__ part, const PcgStatus *status)
+{
+    if (status && status->done) return;
+    constexpr int G = 4, kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    double s[2] = {0.0, 0.0};
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < n && v) {
+            const int32_t b1 = brow_ptr[row + 1];
+            for (int32_t bb = brow_ptr[row] + lane; bb < b1; bb += G) {
+                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+                const double *m = bval + 9 * static_cast<int64_t>(bb);
+                y0 += m[0] * v[j] + m[1] * v[j + 1] + m[2] * v[j + 2];
+                y1 += m[3] * v[j] + m[4] * v[j + 1] + m[5] * v[j + 2];
+                y2 += m[6] * v[j] + m[7] * v[j + 1] + m[8] * v[j + 2];
+            }
+        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
+        if (row < n && lane == 0) {
+            const int64_t o = 3 * row;
+            const double y[3] = {y0, y1, y2};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (MODE == 0) {
+                    out[o + c] = y[c];
+                    s[0] += v[o + c] * y[c];
+                } else {
+                    const double rr = b[o + c] - y[c];
+                    out[o + c] = rr;
+                    u[o + c] = dsc[o + c] * rr;
+                    s[0] += rr * rr;
+                    s[1] += b[o + c] * b[o + c];
+                }
+            }
+        }
+    }
+    write_partials<2>(s, part, kPartStride);
+}
+
+// z = D^-1/2 M u over the n real rows, one wavefront per row (FP32 matrix
+// entries, FP64 products and sums).  IR: x += z.  CG: z stored, partials of
+// z.r_new and z.r_old.
+template <bool kIR>
+__global__ void __launch_bounds__(kThreads)
+bgj_apply_kernel(const float *__restrict__ a, int64_t np, int64_t n,
+                 const double *__restrict__ u, const double *__restrict__ dsc,
+                 double *__restrict__ out, const double *__restrict__ r_new,
+                 const double *__restrict__ r_old, double *__restrict__ part,
+                 const PcgStatus *status)
+{
+    if (status->done) return;
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave0 = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = wave0; i < n; i += nwaves) {
+        const float *row = a + i * np;
+        double s = 0.0;
+        for (int64_t j = 4 * lane; j < np; j += 4 * kWave) {
+            const float4 m4 = *reinterpret_cast<const float4 *>(row + j);
+            s += static_cast<double>(m4.x) * u[j] + static_cast<double>(m4.y) * u[j + 1] +
+                 static_cast<double>(m4.z) * u[j + 2] + static_cast<double>(m4.w) * u[j + 3];
+        }
+        s = wave_sum(s);
+        if (lane == 0) {
+            const double z = dsc[i] * s;
+            if (kIR) {
+                out[i] += z;
+            } else {
+                out[i] = z;
+                v[0] += z * r_new[i];
+                v[1] += z * r_old[i];
+            }
+        }
+    }
+    if (!kIR) write_partials<2>(v, part, kPartStride);
+}
+
+// Stopping test on ||r||^2 partials (and ||b||^2 at init) for the host.
+__global__ void __launch_bounds__(kThreads)
+bgj_check_kernel(const double *__restrict__ part, int32_t np, int32_t k, int32_t cap, int init,
+                 double tol_rel, PcgStatus *status)
+{
+    if (!init && status->done) return;
+    double v[2];
+    sum_partials<2>(part, np, kPartStride, v);
+    if (threadIdx.x != 0) return;
+    if (init) {
+        status->done = 0;
+        status->converged = 0;
+        status->flags = 0;
+        status->refreshes = 0;
+        status->bnorm = sqrt(v[1]);
+        status->tol = tol_rel * sqrt(v[1]);
+    }
+    status->iters = k;
+    status->rnorm = sqrt(v[0]);
+    if (!init) {
+        if (sqrt(v[0]) < status->tol) {
+            status->done = 1;
+            status->converged = 1;
+        } else if (k >= cap) {
+            status->done = 1;
+        }
+    }
+}
+
+// CG: alpha = rz / p.q; x += alpha p (p.q == 0 stops, MatrixOps.c:357).
+__global__ void __launch_bounds__(kThreads)
+bgj_cg_x_kernel(const double *__restrict__ p, double *__restrict__ x, int64_t n,
+                const double *__restrict__ pq_part, int32_t np_pq,
+                const double *__restrict__ rz, PcgStatus *status)
+{
+    if (status->done) return;
+    double pq[2];
+    sum_partials<2>(pq_part, np_pq, kPartStride, pq);
+    if (pq[0] == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->done = 1;
+            status->flags |= 1;
+        }
+        return;
+    }
+    const double alpha = *rz / pq[0];
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        x[i] += alpha * p[i];
+}
+
+// CG: beta = z.(r_new - r_old) / rz_old (0 at init), p = z + beta p,
+// rz_new = z.r_new (workgroup 0 stores it).
+__global__ void __launch_bounds__(kThreads)
+bgj_cg_p_kernel(const double *__restrict__ z, double *__restrict__ p, int64_t n,
+                const double *__restrict__ zr_part, int32_t np_zr, const double *__restrict__ rz_old,
+                double *__restrict__ rz_new, int init, const PcgStatus *status)
+{
+    if (status->done) return;
+    double zr[2];
+    sum_partials<2>(zr_part, np_zr, kPartStride, zr);
+    const double beta = init ? 0.0 : (zr[0] - zr[1]) / *rz_old;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = init ? z[i] : z[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *rz_new = zr[0];
+}
+
+// ---------------------------------------------------------------- drivers
+
+static inline unsigned int nblk(int64_t count, int per = kThreads)
+{
+    return static_cast<unsigned int>((count + per - 1) / per);
+}
+
+int64_t bgj_padded(int64_t n) { return (n + kNb - 1) / kNb * kNb; }
+
+int DenseWork::reserve(int64_t n_)
+{
+    const int64_t np_ = bgj_padded(n_);
+    if (a && np_ <= np_cap) {
+        n = n_;
+        np = np_;
+        return GSLAM_OK;
+    }
+    release();
+    const size_t f = sizeof(float), d = sizeof(double);
+    void *p[8] = {};
+    const size_t sz[8] = {static_cast<size_t>(np_ * np_) * f, static_cast<size_t>(kNb * np_) * f,
+                          static_cast<size_t>(kNb * np_) * f, static_cast<size_t>(kNb * kNb) * d,
+                          static_cast<size_t>(np_) * d, static_cast<size_t>(np_) * d,
+                          static_cast<size_t>(6 * np_ + 8) * d, 256};
+    for (int i = 0; i < 8; ++i) {
+        if (hipMalloc(&p[i], sz[i]) != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
+            set_error("blocked Gauss-Jordan: out of device memory for a " + std::to_string(np_) +
+                      " x " + std::to_string(np_) + " FP32 matrix");
+            return GSLAM_ERR_NOMEM;
+        }
+    }
+    a = static_cast<float *>(p[0]);
+    ct = static_cast<float *>(p[1]);
+    rp = static_cast<float *>(p[2]);
+    pt = static_cast<double *>(p[3]);
+    dsc = static_cast<double *>(p[4]);
+    u = static_cast<double *>(p[5]);
+    vec = static_cast<double *>(p[6]);
+    flag = static_cast<int32_t *>(p[7]);
+    n = n_;
+    np = np_cap = np_;
+    bytes = 0;
+    for (size_t s : sz) bytes += static_cast<int64_t>(s);
+    return GSLAM_OK;
+}
+
+void DenseWork::release()
+{
+    void *p[8] = {a, ct, rp, pt, dsc, u, vec, flag};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    a = ct = rp = nullptr;
+    pt = dsc = u = vec = nullptr;
+    flag = nullptr;
+    n = np = np_cap = bytes = 0;
+}
+
+// One block step's three launches.
+static void bgj_step(hipStream_t s, DenseWork &w, int32_t kp)
+{
+    const int32_t nt = static_cast<int32_t>(w.np / kNb);
+    hipLaunchKernelGGL(bgj_pivot_kernel, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp, w.pt,
+                       w.rp, w.flag);
+    if (nt == 1) return;
+    const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
+    hipLaunchKernelGGL(bgj_panel_kernel, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp, w.pt,
+                       w.rp, w.ct);
+    const int32_t tiles = (nt - 1) * nt;
+    const int32_t per = (tiles + kXcd - 1) / kXcd;
+    hipLaunchKernelGGL(bgj_trailing_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
+                       w.rp, w.np, nt, kp, per);
+}
+
+// In-place FP32 blocked inverse of w.a (np x np, padded).  *singular set on
+// a zero pivot (flag cleared first).
+int bgj_invert(hipStream_t s, DenseWork &w)
+{
+    GSLAM_HIP(hipMemsetAsync(w.flag, 0, sizeof(int32_t), s));
+    const int32_t nt = static_cast<int32_t>(w.np / kNb);
+    for (int32_t kp = 0; kp < nt; ++kp) bgj_step(s, w, kp);
+    GSLAM_HIP(hipGetLastError());
+    return GSLAM_OK;
+}
+
+int bgj_time_trailing(hipStream_t s, DenseWork &w, int reps, hipEvent_t e0, hipEvent_t e1,
+                      double *us)
+{
+    const int32_t nt = static_cast<int32_t>(w.np / kNb);
+    if (nt < 2) {
+        set_error("trailing update timing needs 3N > 128");
+        return GSLAM_ERR_STATE;
+    }
+    const int32_t tiles = (nt - 1) * nt;
+    const int32_t per = (tiles + kXcd - 1) / kXcd;
+    auto launch = [&]() {
+        hipLaunchKernelGGL(bgj_trailing_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
+                           w.rp, w.np, nt, 0, per);
+    };
+    for (int i = 0; i < 3; ++i) launch();
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch();
+    GSLAM_HIP(hipEventRecord(e1, s));
+    GSLAM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *us = 1000.0 * ms / reps;
+    return GSLAM_OK;
+}
+
+int dense_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                      gslam_solve_stats &st)
+{
+    const int64_t n = 3 * static_cast<int64_t>(g.h.n);
+    DenseWork &w = g.dense;
+    GSLAM_TRY(w.reserve(n));
+    const int64_t np = w.np;
+    hipStream_t s = g.stream;
+    const int32_t nrows = g.h.n;
+    double *r0 = w.vec, *r1 = w.vec + np, *z = w.vec + 2 * np, *p = w.vec + 3 * np,
+           *q = w.vec + 4 * np, *rz = w.vec + 6 * np;  // rz[2]
+    double *rbuf[2] = {r0, r1};
+    double *PRs = g.part + kPartOffResid, *PZ = g.part + kPartOffPR0, *PQ = g.part + kPartOffPQ;
+    const unsigned int gs = grid_for(nrows, 4);
+    const unsigned int gv = grid_for(n, 1);
+    const unsigned int ga = static_cast<unsigned int>(std::min<int64_t>(kMaxGrid, (n + 3) / 4));
+    PcgStatus *stp = g.status;
+    const bool cg = o.refine_mode == GSLAM_REFINE_CG;
+    const int32_t cap = std::max(0, o.refine_sweeps);
+
+    GSLAM_HIP(hipEventRecord(g.ev0, s));
+    hipLaunchKernelGGL(bgj_clear_kernel, dim3(nblk(np * np / 4)), dim3(kThreads), 0, s, w.a, n, np);
+    hipLaunchKernelGGL(bgj_scale_bsr_kernel, dim3(nblk(np)), dim3(kThreads), 0, s, g.bdiag, g.bval,
+                       n, np, w.dsc);
+    hipLaunchKernelGGL(bgj_scatter_bsr_kernel, dim3(nblk(9 * g.h.n_blocks)), dim3(kThreads), 0, s,
+                       g.brow_ptr, g.bcol, g.bval, g.h.n_blocks, nrows, w.dsc, np, w.a);
+    GSLAM_TRY(bgj_invert(s, w));
+    GSLAM_HIP(hipEventRecord(g.ev2, s));
+
+    // x = 0, r0 = b, u = D^-1/2 b (padding of u zero), ||b||
+    GSLAM_HIP(hipMemsetAsync(w.u, 0, static_cast<size_t>(np) * sizeof(double), s));
+    GSLAM_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(double), s));
+    hipLaunchKernelGGL(bgj_spmv_kernel<1>, dim3(gs), dim3(kThreads), 0, s, g.brow_ptr, g.bcol,
+                       g.bval, static_cast<const double *>(nullptr), b, w.dsc, r0, w.u, nrows, PRs,
+                       static_cast<const PcgStatus *>(nullptr));
+    hipLaunchKernelGGL(bgj_check_kernel, dim3(1), dim3(kThreads), 0, s, PRs,
+                       static_cast<int32_t>(gs), 0, cap, 1, o.tol, stp);
+    int32_t k = 0;
+    gslam_solve_opts lo = o;
+    std::function<void(int32_t)> enqueue;
+    if (!cg) {
+        // sweep k = 0 is the FP32 solve x = M b, sweeps 1..cap refine
+        lo.max_iter = cap + 1;
+        enqueue = [&](int32_t m) {
+            for (int32_t c = 0; c < m; ++c, ++k) {
+                hipLaunchKernelGGL(bgj_apply_kernel<true>, dim3(ga), dim3(kThreads), 0, s, w.a, np,
+                                   n, w.u, w.dsc, x, static_cast<const double *>(nullptr),
+                                   static_cast<const double *>(nullptr), PZ, stp);
+                hipLaunchKernelGGL(bgj_spmv_kernel<1>, dim3(gs), dim3(kThreads), 0, s, g.brow_ptr,
+                                   g.bcol, g.bval, x, b, w.dsc, r0, w.u, nrows, PRs, stp);
+                hipLaunchKernelGGL(bgj_check_kernel, dim3(1), dim3(kThreads), 0, s, PRs,
+                                   static_cast<int32_t>(gs), k, cap, 0, o.tol, stp);
+            }
+        };
+    } else {
+        // z0 = M r0, p = z0, rz = z0.r0
+        hipLaunchKernelGGL(bgj_apply_kernel<false>, dim3(ga), dim3(kThreads), 0, s, w.a, np, n,
+                           w.u, w.dsc, z, r0, r0, PZ, stp);
+        hipLaunchKernelGGL(bgj_cg_p_kernel, dim3(gv), dim3(kThreads), 0, s, z, p, n, PZ,
+                           static_cast<int32_t>(ga), rz, rz + 1, 1, stp);
+        lo.max_iter = cap;
+        k = 0;
+        enqueue = [&](int32_t m) {
+            for (int32_t c = 0; c < m; ++c) {
+                ++k;
+                double *rnew = rbuf[k & 1], *rold = rbuf[(k + 1) & 1];
+                hipLaunchKernelGGL(bgj_spmv_kernel<0>, dim3(gs), dim3(kThreads), 0, s, g.brow_ptr,
+                                   g.bcol, g.bval, p, static_cast<const double *>(nullptr),
+                                   w.dsc, q, static_cast<double *>(nullptr), nrows, PQ, stp);
+                hipLaunchKernelGGL(bgj_cg_x_kernel, dim3(gv), dim3(kThreads), 0, s, p, x, n, PQ,
+                                   static_cast<int32_t>(gs), rz + (k & 1), stp);
+                hipLaunchKernelGGL(bgj_spmv_kernel<1>, dim3(gs), dim3(kThreads), 0, s, g.brow_ptr,
+                                   g.bcol, g.bval, x, b, w.dsc, rnew, w.u, nrows, PRs, stp);
+                hipLaunchKernelGGL(bgj_check_kernel, dim3(1), dim3(kThreads), 0, s, PRs,
+                                   static_cast<int32_t>(gs), k, cap, 0, o.tol, stp);
+                hipLaunchKernelGGL(bgj_apply_kernel<false>, dim3(ga), dim3(kThreads), 0, s, w.a,
+                                   np, n, w.u, w.dsc, z, rnew, rold, PZ, stp);
+                hipLaunchKernelGGL(bgj_cg_p_kernel, dim3(gv), dim3(kThreads), 0, s, z, p, n, PZ,
+                                   static_cast<int32_t>(ga), rz + (k & 1), rz + ((k + 1) & 1), 0,
+                                   stp);
+            }
+        };
+    }
+    if (cg && cap == 0) {
+        // no refinement: x = M b
+        hipLaunchKernelGGL(bgj_apply_kernel<true>, dim3(ga), dim3(kThreads), 0, s, w.a, np, n, w.u,
+                           w.dsc, x, static_cast<const double *>(nullptr),
+                           static_cast<const double *>(nullptr), PZ, stp);
+    } else {
+        GSLAM_TRY(run_chunked(g, lo, enqueue, k));
+    }
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev1, s));
+    GSLAM_HIP(hipMemcpyAsync(g.status_host, stp, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+    int32_t singular = 0;
+    GSLAM_HIP(hipMemcpyAsync(g.scal_host + 3, w.flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GSLAM_HIP(hipStreamSynchronize(s));
+    std::memcpy(&singular, g.scal_host + 3, sizeof(int32_t));
+    float ms = 0.f, msf = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    GSLAM_HIP(hipEventElapsedTime(&msf, g.ev0, g.ev2));
+    const PcgStatus &ps = *g.status_host;
+    st.ms = ms;
+    st.ms_factor = msf;
+    st.iters = ps.iters;
+    st.converged = ps.converged;
+    st.flags = ps.flags;
+    st.reliable_updates = 0;
+    GSLAM_TRY(true_rel_residual(g, b, x, &st.true_rel_res));
+    st.rel_res = st.true_rel_res;
+    if (singular) {
+        set_error("blocked Gauss-Jordan: zero pivot");
+        GSLAM_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(double), s));
+        GSLAM_HIP(hipStreamSynchronize(s));
+        return GSLAM_ERR_SINGULAR;
+    }
+    return GSLAM_OK;
+}
+
+// Host matrix in, host FP64 inverse out (FP32 accuracy, no refinement).
+int bgj_invert_host(const double *m_host, double *inv_host, int64_t n)
+{
+    hipStream_t s = nullptr;
+    GSLAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DenseWork w;
+    double *m = nullptr;
+    int rc = w.reserve(n);
+    if (rc == GSLAM_OK && hipMalloc(reinterpret_cast<void **>(&m), static_cast<size_t>(n * n) * 8) != hipSuccess) {
+        set_error("gslam_gj_invert: out of device memory");
+        rc = GSLAM_ERR_NOMEM;
+    }
+    int32_t singular = 0;
+    if (rc == GSLAM_OK) {
+        const int64_t np = w.np;
+        if (hipMemcpyAsync(m, m_host, static_cast<size_t>(n * n) * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = GSLAM_ERR_HIP;
+        } else {
+            hipLaunchKernelGGL(bgj_scale_dense_kernel, dim3(nblk(np)), dim3(kThreads), 0, s, m, n,
+                               np, w.dsc);
+            hipLaunchKernelGGL(bgj_load_dense_kernel, dim3(nblk(np * np)), dim3(kThreads), 0, s, m,
+                               n, np, w.dsc, w.a);
+            rc = bgj_invert(s, w);
+        }
+        if (rc == GSLAM_OK) {
+            hipLaunchKernelGGL(bgj_store_dense_kernel, dim3(nblk(n * n)), dim3(kThreads), 0, s, w.a,
+                               n, np, w.dsc, w.flag, m);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(inv_host, m, static_cast<size_t>(n * n) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(&singular, w.flag, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                set_error("gslam_gj_invert: HIP error");
+                rc = GSLAM_ERR_HIP;
+            }
+        }
+    }
+    (void)hipFree(m);
+    w.release();
+    (void)hipStreamDestroy(s);
+    if (rc == GSLAM_OK && singular) {
+        set_error("Gauss-Jordan: zero pivot (the inverse is the zero matrix)");
+        return GSLAM_ERR_SINGULAR;
+    }
+    return rc;
+}
+
+}  // namespace gslam

@@ -44,6 +44,20 @@ struct PcgStatus {
     double tol;          // tol_rel * ||b||
 };
 
+// Work space of the blocked Gauss-Jordan (GJ_MIXED): the padded FP32 matrix
+// (np x np, np = 3N rounded up to the 128-wide panel), the k-major column
+// and row panels, the FP64 pivot block, the Jacobi scaling and refinement
+// vectors.  Kept between solves; grown on demand.
+struct DenseWork {
+    float *a = nullptr, *ct = nullptr, *rp = nullptr;
+    double *pt = nullptr, *dsc = nullptr, *u = nullptr, *vec = nullptr;
+    int32_t *flag = nullptr;
+    int64_t n = 0, np = 0, np_cap = 0, bytes = 0;
+    int reserve(int64_t n_);
+    void release();
+    ~DenseWork() { release(); }
+};
+
 struct DeviceGraph {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -88,13 +102,15 @@ struct DeviceGraph {
     PcgStatus *status_ring = nullptr;  // pinned, 2 slots for look-ahead polling
     double *scal = nullptr;      // small device scalars (diff etc.)
     double *scal_host = nullptr; // pinned
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_poll[2] = {nullptr, nullptr};
 
     // factorized FP32 operator of the mixed solver: per incidence entry k
     // (pose i, incident edge e, side a; layout of xlist) the other pose
     // xoth[k] and 8 FP32 edge factors frec[8k..8k+7]; xpos[2e+a] = k.
     int32_t *xoth = nullptr, *xpos = nullptr;
     float *frec = nullptr;
+
+    DenseWork dense;             // GJ_MIXED work space (allocated on first use)
 
     std::vector<void *> allocs;
 
@@ -133,6 +149,16 @@ int launch_csr_to_bsr(DeviceGraph &g, const double *csr_vals, double *bval);
 // ---- gj.hip ----
 int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
                 gslam_solve_stats &s);
+
+// ---- gj_blocked.hip ----
+int64_t bgj_padded(int64_t n);
+int dense_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                      gslam_solve_stats &s);
+// FP32 blocked inverse of a host n x n FP64 matrix, returned as FP64.
+int bgj_invert_host(const double *m_host, double *inv_host, int64_t n);
+// Mean duration of the trailing-update kernel of block step 0 on w.
+int bgj_time_trailing(hipStream_t s, DenseWork &w, int reps, hipEvent_t e0, hipEvent_t e1,
+                      double *us);
 
 // ---- pcg.hip ----
 int pcg_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b,

@@ -23,7 +23,8 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_STATE, ERR_TOO_LARGE, ERR_NOT_CONVERGED = (
     -1, -2, -3, -4, -5, -6, -7)
 SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3}
-KERNELS = {"spmv64": 0, "update64": 1, "spmv_mixed": 2, "assembly": 3}
+KERNELS = {"spmv64": 0, "update64": 1, "spmv_mixed": 2, "assembly": 3, "gj_trailing": 4}
+REFINE = {"ir": 0, "cg": 1}
 
 
 class GSlamError(RuntimeError):
@@ -54,13 +55,13 @@ class _Info(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("refine_sweeps", C.c_int32), ("refresh_every", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("refine_mode", C.c_int32)]
 
 
 class _Stats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("reliable_updates", C.c_int32),
                 ("flags", C.c_int32), ("rel_res", C.c_double), ("true_rel_res", C.c_double),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("ms_factor", C.c_double)]
 
 
 _LIB = None
@@ -164,7 +165,7 @@ def read_log(path: str) -> dict:
 
 
 def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
-              check_every=0) -> _Opts:
+              check_every=0, refine_mode="ir") -> _Opts:
     o = _Opts()
     lib().gslam_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -173,12 +174,14 @@ def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_e
     o.refine_sweeps = refine_sweeps
     o.refresh_every = refresh_every
     o.check_every = check_every
+    o.refine_mode = REFINE[refine_mode] if isinstance(refine_mode, str) else int(refine_mode)
     return o
 
 
 def _stats(s: _Stats) -> dict:
     return dict(iters=s.iters, converged=bool(s.converged), reliable_updates=s.reliable_updates,
-                flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms)
+                flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms,
+                ms_factor=s.ms_factor)
 
 
 class Graph:
@@ -348,7 +351,10 @@ class Graph:
 
 
 def gj_invert(a, solver="gj64") -> np.ndarray:
-    """Dense unpivoted Gauss-Jordan inverse on the GPU (host in, host out)."""
+    """Dense unpivoted Gauss-Jordan inverse on the GPU (host in, host out).
+
+    "gj64": FP64, one pass per pivot; "gj_mixed": blocked, FP32 MFMA
+    trailing updates on the Jacobi-scaled matrix (FP32 accuracy)."""
     a = np.ascontiguousarray(a, np.float64)
     n = a.shape[0]
     out = np.zeros_like(a)

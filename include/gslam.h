@@ -105,16 +105,30 @@ typedef struct {
     int32_t refresh_every;      /* PCG_MIXED: FP64 true-residual refresh and
                                    convergence test every k iterations (64)  */
     int32_t check_every;        /* PCG: host polls convergence every k iters */
+    int32_t refine_mode;        /* GJ_MIXED: GSLAM_REFINE_IR or _CG          */
 } gslam_solve_opts;
 
+/* GJ_MIXED refinement on the true residual xi - Omega dx (FP64), with the
+ * FP32 blocked inverse M of the Jacobi-scaled Omega as the approximate
+ * solver; both stop early once ||xi - Omega dx|| < tol ||xi||.
+ *   IR: dx += D^-1/2 M D^-1/2 (xi - Omega dx), refine_sweeps times after the
+ *       first FP32 solve;
+ *   CG: up to refine_sweeps conjugate-gradient iterations preconditioned by
+ *       the same operator (flexible beta; for ill-conditioned Omega, where the
+ *       FP32 inverse alone has no correct digit, this still converges). */
+#define GSLAM_REFINE_IR 0
+#define GSLAM_REFINE_CG 1
+
 typedef struct {
-    int32_t iters;              /* PCG iterations executed                   */
+    int32_t iters;              /* PCG iterations / GJ_MIXED refinement
+                                   iterations executed                      */
     int32_t converged;          /* 1 if the tolerance was met                */
     int32_t reliable_updates;   /* PCG_MIXED: FP64 residual refreshes        */
     int32_t flags;              /* bit0: p.q == 0 break (MatrixOps.c:357)     */
     double rel_res;             /* final ||r||/||xi|| of the solver's residual */
     double true_rel_res;        /* ||xi - Omega dx|| / ||xi||, FP64          */
     double ms;                  /* device time of the solve (HIP events)     */
+    double ms_factor;           /* GJ: device time of the dense inversion     */
 } gslam_solve_stats;
 
 /* Fills opts with the reference defaults (PCG FP64, tol 1e-6, 500 iterations:
@@ -193,6 +207,9 @@ int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
 #define GSLAM_KERNEL_UPDATE_FP64 1  /* FP64 PCG: x, r, z update + partials      */
 #define GSLAM_KERNEL_SPMV_MIXED  2  /* mixed PCG: factorized FP32 operator      */
 #define GSLAM_KERNEL_ASSEMBLY    3  /* edge evaluation + Omega/xi gathers       */
+#define GSLAM_KERNEL_GJ_TRAILING 4  /* GJ_MIXED: FP32 MFMA trailing update of
+                                       the first block step (needs a GJ_MIXED
+                                       solve first; overwrites its workspace)  */
 int gslam_time_kernel(gslam_graph *g, int32_t which, int32_t reps, double *us_per_launch);
 
 /* Last error message of the calling thread ("" if none). */
