@@ -42,7 +42,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNb = 128;          // panel width, trailing tile edge and GEMM depth
-constexpr int kPivThreads = 1024; // pivot kernel: 128 rows x 8 column phases
+constexpr int kPivThreads = 256;  // pivot kernel: 16 x 16 threads, 8 x 8 entries each
 constexpr int kXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // ---------------------------------------------------------------- loading
@@ -143,61 +143,86 @@ bgj_store_dense_kernel(const float *__restrict__ a, int64_t n, int64_t np,
 
 // ---------------------------------------------------------------- block step
 
-// P = A_KK^-1 in FP64 by in-place Gauss-Jordan.  Thread t owns row t/8 and
-// columns (t%8) + 8q, q < 16, in registers; per pivot the owners publish the
-// pivot row and column to LDS.  Writes A_KK = P and R_K = P (FP32) and
-// PT = P^T (FP64) for the row-panel kernel.  A zero pivot sets *singular
-// (the whole inverse is then reported as zero, MatrixOps.c:60-68).
+// P = A_KK^-1 in FP64 by in-place Gauss-Jordan in one workgroup.  Thread t
+// owns the 8 x 8 sub-block (rows 8(t/16).., columns 8(t%16)..) in registers.
+// Per pivot p the owners publish row p and column p to LDS (double-buffered
+// by pivot parity, so one barrier per pivot suffices), the column owners
+// then set their column-p entries to 0 (row p: 1) and publish 1 in place of
+// the pivot in the row buffer, so that every row r != p takes the same form
+//   m[r][c] -= (col[r] / piv) * row[c]
+// (giving -col[r]/piv in column p) and row p is scaled by 1/piv.  Writes
+// A_KK = P and R_K = P (FP32) and PT = P^T (FP64) for the row-panel kernel.
+// A zero pivot sets *singular (the whole inverse is then reported as zero,
+// MatrixOps.c:60-68).
 __global__ void __launch_bounds__(kPivThreads)
 bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
                  float *__restrict__ rp, int32_t *__restrict__ singular)
 {
-    __shared__ double rowbuf[kNb], colbuf[kNb];
+    __shared__ double rowbuf[2][kNb], colbuf[2][kNb], pivbuf[2];
     const int t = threadIdx.x;
-    const int r = t >> 3, c0 = t & 7;
+    const int rb = t >> 4, cb = t & 15;
     const int64_t k0 = static_cast<int64_t>(kp) * kNb;
-    double m[16];
-    const float *src = a + (k0 + r) * np + k0 + c0;
+    double m[8][8];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) m[q] = static_cast<double>(src[8 * q]);
+    for (int i = 0; i < 8; ++i) {
+        const float *src = a + (k0 + 8 * rb + i) * np + k0 + 8 * cb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m[i][j] = static_cast<double>(src[j]);
+    }
     bool zero = false;
+    for (int pb = 0; pb < kNb / 8; ++pb) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        for (int s = 0; s < 8; ++s) {
-            const int p = 8 * q + s;
-            if (r == p) {
+        for (int pi = 0; pi < 8; ++pi) {
+            const int buf = pi & 1;
+            if (rb == pb) {
 #pragma unroll
-                for (int qq = 0; qq < 16; ++qq) rowbuf[c0 + 8 * qq] = m[qq];
+                for (int j = 0; j < 8; ++j)
+                    rowbuf[buf][8 * cb + j] = (j == pi && cb == pb) ? 1.0 : m[pi][j];
             }
-            if (c0 == s) colbuf[r] = m[q];
+            if (cb == pb) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) colbuf[buf][8 * rb + i] = m[i][pi];
+                if (rb == pb) pivbuf[buf] = m[pi][pi];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) m[i][pi] = (i == pi && rb == pb) ? 1.0 : 0.0;
+            }
             __syncthreads();
-            double piv = rowbuf[p];
+            double piv = pivbuf[buf];
             if (piv == 0.0) {
                 zero = true;
                 piv = 1.0;
             }
             const double ip = 1.0 / piv;
-            if (r == p) {
+            double rv[8], cv[8];
 #pragma unroll
-                for (int qq = 0; qq < 16; ++qq) m[qq] = (c0 + 8 * qq == p) ? ip : m[qq] * ip;
-            } else {
-                const double f = colbuf[r] * ip;
+            for (int j = 0; j < 8; ++j) rv[j] = rowbuf[buf][8 * cb + j];
 #pragma unroll
-                for (int qq = 0; qq < 16; ++qq)
-                    m[qq] = (c0 + 8 * qq == p) ? -f : m[qq] - f * rowbuf[c0 + 8 * qq];
+            for (int i = 0; i < 8; ++i) cv[i] = colbuf[buf][8 * rb + i];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i == pi && rb == pb) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) m[i][j] *= ip;
+                } else {
+                    const double f = cv[i] * ip;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) m[i][j] = fma(-f, rv[j], m[i][j]);
+                }
             }
-            __syncthreads();
         }
     }
-    if (zero && c0 == 0) *singular = 1;
-    float *dst = a + (k0 + r) * np + k0 + c0;
-    float *rdst = rp + static_cast<int64_t>(r) * np + k0 + c0;
+    if (zero && t == 0) *singular = 1;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int c = c0 + 8 * q;
-        dst[8 * q] = static_cast<float>(m[q]);
-        rdst[8 * q] = static_cast<float>(m[q]);
-        pt[c * kNb + r] = m[q];
+    for (int i = 0; i < 8; ++i) {
+        const int r = 8 * rb + i;
+        float *dst = a + (k0 + r) * np + k0 + 8 * cb;
+        float *rdst = rp + static_cast<int64_t>(r) * np + k0 + 8 * cb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            dst[j] = static_cast<float>(m[i][j]);
+            rdst[j] = static_cast<float>(m[i][j]);
+            pt[(8 * cb + j) * kNb + r] = m[i][j];
+        }
     }
 }
 
@@ -312,6 +337,8 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
     }
     const float *pa = ct + static_cast<int64_t>(h) * np + i0 + l32;
     const float *pb = rp + static_cast<int64_t>(h) * np + j0 + l32;
+    // (an explicit 8-step-deep operand prefetch measured 10% slower than
+    // this loop, whose loads the compiler issues one step ahead)
 #pragma unroll 8
     for (int s = 0; s < kNb / 2; ++s) {
         const int64_t o = 2 * static_cast<int64_t>(s) * np;
