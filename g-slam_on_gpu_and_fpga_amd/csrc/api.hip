@@ -65,6 +65,9 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     g.snr[0] = d.snr[0];
     g.snr[1] = d.snr[1];
     g.wrap = d.wrap ? 1 : 0;
+    g.anchor_prior = d.anchor_prior ? 1 : 0;
+    if (d.poses && d.n_poses > 0)
+        for (int k = 0; k < 3; ++k) g.pose0_ref[k] = d.poses[k];
 
     GSLAM_HIP(hipGetDevice(&g.dev));
     GSLAM_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));

@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kThreads)
 between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
                      const int32_t *__restrict__ t2, const double *__restrict__ bp,
                      const double *__restrict__ bw, int64_t n_bet, int64_t e_off,
-                     double lambda, int wrap, double *__restrict__ stage,
+                     double lambda, int wrap, int exact, double *__restrict__ stage,
                      float *__restrict__ frec, const int32_t *__restrict__ xpos)
 {
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -230,6 +230,24 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
         st[36 + r] = -(Ji[r] * L[0] * ev[0] + Ji[3 + r] * L[1] * ev[1] + Ji[6 + r] * L[2] * ev[2]);
         st[39 + r] = -(Jj[r] * L[0] * ev[0] + Jj[3 + r] * L[1] * ev[1] + Jj[6 + r] * L[2] * ev[2]);
     }
+    if (exact) {
+        // second-order terms sum_k (L e)_k d2 e_k of the exact Hessian: the
+        // translation residual R(th_i)^T (t_j - t_i) is linear in t and
+        // quadratic only through th_i, so they touch the th_i row and column:
+        // (th_i, th_i) c_tt, (th_i, t_i) (cx, cy), (th_i, t_j) -(cx, cy)
+        const double g0 = L0 * e0, g1 = L1 * e1;
+        const double c_tt = -g0 * (c * dx + s * dy) + g1 * (s * dx - c * dy);
+        const double cx = g0 * s + g1 * c, cy = -g0 * c + g1 * s;
+        st[2] += cx;
+        st[5] += cy;
+        st[6] += cx;
+        st[7] += cy;
+        st[8] += c_tt;
+        st[9 + 6] -= cx;
+        st[9 + 7] -= cy;
+        st[18 + 2] -= cx;
+        st[18 + 5] -= cy;
+    }
     // factors: rotation (c, s), theta column (ga, gb), W = diag(L)
     put_factors(frec, xpos, e,
                 make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(ga),
@@ -258,9 +276,13 @@ gather_blocks_kernel(const int32_t *__restrict__ cptr, const int32_t *__restrict
     if (bval32) bval32[t] = static_cast<float>(v);
 }
 
+// xi entries: contributions in edge order; with the anchor prior, pose 0
+// also gets anchor * (ref - pose0) (the reference has no xi anchor term).
 __global__ void __launch_bounds__(kThreads)
 gather_xi_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ xlist,
-                 const double *__restrict__ stage, int64_t dim, double *__restrict__ xi)
+                 const double *__restrict__ stage, int64_t dim, double *__restrict__ xi,
+                 const double *__restrict__ poses, int prior, double anchor, double ref0,
+                 double ref1, double ref2)
 {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= dim) return;
@@ -272,6 +294,7 @@ gather_xi_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ x
         const int32_t ent = xlist[c];
         v += stage[static_cast<int64_t>(ent >> 1) * kStage + 36 + (ent & 1) * 3 + r];
     }
+    if (prior && i == 0) v += anchor * ((r == 0 ? ref0 : r == 1 ? ref1 : ref2) - poses[r]);
     xi[t] = v;
 }
 
@@ -299,7 +322,7 @@ csr_to_bsr_kernel(const int32_t *__restrict__ pos, const double *__restrict__ cs
 
 static inline unsigned int nblk(int64_t n) { return static_cast<unsigned int>((n + kThreads - 1) / kThreads); }
 
-int launch_assembly(DeviceGraph &g)
+int launch_assembly(DeviceGraph &g, bool exact_hessian)
 {
     const HostGraph &h = g.h;
     if (h.n_obs_e > 0)
@@ -316,14 +339,16 @@ int launch_assembly(DeviceGraph &g)
     if (h.n_bet_e > 0)
         hipLaunchKernelGGL(between_edges_kernel, dim3(nblk(h.n_bet_e)), dim3(kThreads), 0,
                            g.stream, g.poses, g.e_t1, g.e_t2, g.bet_param, g.bet_w, h.n_bet_e,
-                           h.n_obs_e + h.n_mot_e, g.lambda, g.wrap, g.stage, g.frec, g.xpos);
+                           h.n_obs_e + h.n_mot_e, g.lambda, g.wrap, exact_hessian ? 1 : 0, g.stage,
+                           g.frec, g.xpos);
     const int64_t nv = 9 * h.n_blocks;
     hipLaunchKernelGGL(gather_blocks_kernel, dim3(nblk(nv)), dim3(kThreads), 0, g.stream,
                        g.cptr, g.clist, g.stage, nv, static_cast<int64_t>(h.bdiag[0]), g.anchor,
                        g.bval, g.bval32);
     const int64_t dim = 3 * static_cast<int64_t>(h.n);
     hipLaunchKernelGGL(gather_xi_kernel, dim3(nblk(dim)), dim3(kThreads), 0, g.stream, g.xptr,
-                       g.xlist, g.stage, dim, g.xi);
+                       g.xlist, g.stage, dim, g.xi, g.poses, g.anchor_prior, g.anchor,
+                       g.pose0_ref[0], g.pose0_ref[1], g.pose0_ref[2]);
     GSLAM_HIP(hipGetLastError());
     return GSLAM_OK;
 }

@@ -65,6 +65,8 @@ struct DeviceGraph {
     double lambda = 0.1, anchor = 1e6, delta = 1.0, mns[4] = {0.19, 0.001, 0.13, 0.2};
     double snr[2] = {0.14, 0.05};
     int wrap = 0;
+    int anchor_prior = 0;
+    double pose0_ref[3] = {0.0, 0.0, 0.0};  // anchor prior reference (pose 0 at create)
     int64_t bytes = 0;
     bool assembled = false, solved = false;
     bool factors_valid = false;  // frec matches the assembled system
@@ -140,11 +142,16 @@ int DeviceGraph::alloc(T **ptr, int64_t count)
 
 // ---- assembly.hip ----
 // Evaluates every edge at the current poses and gathers Omega (BSR, FP64 and
-// FP32 copy) and xi.
-int launch_assembly(DeviceGraph &g);
+// FP32 copy) and xi.  exact_hessian: between edges add their second-order
+// terms (the exact Hessian of the cost instead of the Gauss-Newton matrix).
+int launch_assembly(DeviceGraph &g, bool exact_hessian = false);
 // BSR values -> reference CSR values (device to device).
 int launch_bsr_to_csr(DeviceGraph &g, const double *bval, double *csr_vals);
 int launch_csr_to_bsr(DeviceGraph &g, const double *csr_vals, double *bval);
+
+// ---- adjoint.hip ----
+// dl_dw[k] = -(J_k v)^T L_k e_k for every between edge k (device arrays).
+int launch_adjoint_grad(DeviceGraph &g, const double *v, double *dl_dw);
 
 // ---- gj.hip ----
 int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,

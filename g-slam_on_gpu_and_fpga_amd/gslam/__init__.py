@@ -43,7 +43,7 @@ class _Desc(C.Structure):
         ("snr", C.c_double * 2),
         ("n_between", C.c_int64), ("be_i", C.c_void_p), ("be_j", C.c_void_p),
         ("be_meas", C.c_void_p), ("be_info", C.c_void_p),
-        ("anchor", C.c_double), ("wrap", C.c_int32)]
+        ("anchor", C.c_double), ("wrap", C.c_int32), ("anchor_prior", C.c_int32)]
 
 
 class _Info(C.Structure):
@@ -73,6 +73,7 @@ EXPORTS = [
     "gslam_set_log_weights", "gslam_assemble", "gslam_get_system_csr", "gslam_get_system_bsr",
     "gslam_solve", "gslam_get_dx", "gslam_solve_rhs", "gslam_apply_update", "gslam_gn_step",
     "gslam_gj_invert", "gslam_last_error", "gslam_set_system_csr", "gslam_time_kernel",
+    "gslam_adjoint",
     "device_graph_init", "device_graph_update", "device_graph_get_row_ptr",
     "device_graph_get_col_idx", "device_build_diag_inv", "device_pcg_solve",
     "device_graph_free", "inv_cuda",
@@ -189,7 +190,7 @@ class Graph:
 
     def __init__(self, poses, *, motion=False, u_nu=None, u_omega=None, delta=1.0, lam=0.1,
                  mns=(0.19, 0.001, 0.13, 0.2), obs=None, pair_order=0, snr=(0.14, 0.05),
-                 between=None, anchor=1e6, wrap=0):
+                 between=None, anchor=1e6, wrap=0, anchor_prior=0):
         L = lib()
         poses = _f64(poses)
         n = poses.size // 3
@@ -224,6 +225,7 @@ class Graph:
             d.be_i, d.be_j, d.be_meas, d.be_info = _p(bi), _p(bj), _p(bm), _p(bf)
         d.anchor = anchor
         d.wrap = wrap
+        d.anchor_prior = anchor_prior
         h = C.c_void_p()
         _check(L.gslam_create(C.byref(d), C.byref(h)), "gslam_create")
         self._keep = []
@@ -321,6 +323,28 @@ class Graph:
         _check(lib().gslam_solve(self.h, C.byref(o), C.byref(s)), "gslam_solve",
                allow=(ERR_NOT_CONVERGED, ERR_SINGULAR))
         return _stats(s)
+
+    def adjoint(self, dl_dx, tol=1e-12, max_iter=200000):
+        """dL/dw_e (between edges) and dL/dlambda at the current (converged)
+        poses for the caller's dL/dx*: one FP64 PCG on the exact Hessian and an
+        edge-parallel gradient kernel."""
+        o = make_opts("pcg64", tol, max_iter)
+        g = _f64(dl_dx, 3 * self.n)
+        dw = np.zeros(self.info["n_between_edges"])
+        dlam = C.c_double()
+        s = _Stats()
+        _check(lib().gslam_adjoint(self.h, _p(g), C.byref(o), _p(dw), C.byref(dlam), C.byref(s)),
+               "gslam_adjoint")
+        return dw, dlam.value, _stats(s)
+
+    def converge(self, solver="pcg64", tol=1e-12, max_iter=200000, diff_tol=1e-12,
+                 max_rounds=50, **kw) -> int:
+        """Gauss-Newton steps until ||dx||/sqrt(3N) < diff_tol; returns rounds."""
+        for r in range(1, max_rounds + 1):
+            _, diff = self.gn_step(solver, tol, max_iter, **kw)
+            if diff < diff_tol:
+                return r
+        return max_rounds
 
     def dx(self) -> np.ndarray:
         out = np.zeros(3 * self.n)

@@ -79,6 +79,12 @@ typedef struct {
     int32_t wrap;               /* 0: reference semantics (no angle wrap);
                                    1: wrap theta residuals and updated poses
                                    to [-pi, pi)                               */
+    int32_t anchor_prior;       /* 0: the anchor enters Omega only (reference,
+                                   MatrixOps.c:223-231); 1: it is a prior on
+                                   pose 0 at its value in `poses` at
+                                   gslam_create, adding anchor * (ref - pose0)
+                                   to xi, so the converged poses are a smooth
+                                   function of the edge weights (gslam_adjoint) */
 } gslam_graph_desc;
 
 typedef struct {
@@ -199,6 +205,22 @@ int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
  * no refinement of the inverse itself).  GSLAM_ERR_SINGULAR on a zero pivot
  * (inv is then all zeros, as in the reference). */
 int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
+
+/* Adjoint of the converged poses x* of a between-edge graph (no reference
+ * counterpart; the reference tunes lambda by a manual grid search,
+ * README.md:137-163).  With f = 1/2 sum_e e_e^T L_e e_e (+ anchor prior),
+ * L_e = exp(w_e) lambda diag(info_e), and the caller's dL/dx* (3N doubles),
+ * it assembles the exact Hessian H of f at the current poses (Gauss-Newton
+ * blocks plus the second-order terms of the rotated translation residual,
+ * same block pattern as Omega), solves H v = dL/dx* with the FP64 PCG
+ * (opts->tol, opts->max_iter; the solver field is ignored), then an
+ * edge-parallel kernel forms
+ *   dL/dw_e    = -(J_e v)^T L_e e_e            (n_between doubles, dl_dw)
+ *   dL/dlambda = sum_e dL/dw_e / lambda          (dl_dlambda)
+ * Requires desc.anchor_prior = 1, converged poses, and between edges only.
+ * Leaves the handle unassembled (the next solve assembles again). */
+int gslam_adjoint(gslam_graph *g, const double *dl_dx, const gslam_solve_opts *opts,
+                  double *dl_dw, double *dl_dlambda, gslam_solve_stats *stats);
 
 /* Kernel timing on the current system: runs one kernel `reps` times back to
  * back on the handle's stream and returns the mean duration per launch in
