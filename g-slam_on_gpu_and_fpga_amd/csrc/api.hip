@@ -101,6 +101,7 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.csr_pos, 9 * B));
     g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
     GSLAM_TRY(g.alloc(&g.tile_row, h.tile_row.size()));
+    GSLAM_TRY(g.alloc(&g.tile_info, g.n_tiles));
     GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
     GSLAM_TRY(g.alloc(&g.bval32, 9 * B));
@@ -147,6 +148,14 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.upload(g.csr_col_idx, h.csr_col_idx.data(), h.nnz * 4));
     GSLAM_TRY(g.upload(g.csr_pos, h.csr_pos.data(), 9 * B * 4));
     GSLAM_TRY(g.upload(g.tile_row, h.tile_row.data(), h.tile_row.size() * 4));
+    {
+        std::vector<int4> ti(static_cast<size_t>(g.n_tiles));
+        for (int32_t t = 0; t < g.n_tiles; ++t) {
+            const int32_t r0 = h.tile_row[t], r1 = h.tile_row[t + 1];
+            ti[t] = make_int4(r0, r1, h.brow_ptr[r0], h.brow_ptr[r1]);
+        }
+        GSLAM_TRY(g.upload(g.tile_info, ti.data(), static_cast<int64_t>(ti.size()) * 16));
+    }
     GSLAM_TRY(g.upload(g.xoth, h.xoth.data(), 2 * E * 4));
     GSLAM_TRY(g.upload(g.xpos, h.xpos.data(), 2 * E * 4));
     return GSLAM_OK;

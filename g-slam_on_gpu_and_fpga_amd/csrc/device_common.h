@@ -50,6 +50,29 @@ __device__ inline void block_sum(double (&v)[K], double (*lds)[NT / kWave])
     __syncthreads();
 }
 
+// Workgroup sum of K values that every thread receives, with one barrier:
+// wave butterflies, per-wave totals to LDS, then each thread adds the totals
+// in wave order (the same bits in every thread and every workgroup that sums
+// the same inputs).  lds must not be reused before another barrier.
+template <int K, int NT>
+__device__ inline void block_allsum(double (&v)[K], double (*lds)[NT / kWave])
+{
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (l == 0) lds[k][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NT / kWave; ++i) s += lds[k][i];
+        v[k] = s;
+    }
+}
+
 // Angle wrap to [-pi, pi): a - 2 pi floor((a + pi) / 2 pi).
 __device__ inline double wrap_pi(double a)
 {

@@ -83,6 +83,7 @@ struct DeviceGraph {
     int32_t *cptr = nullptr, *clist = nullptr, *xptr = nullptr, *xlist = nullptr;
     int32_t *csr_row_ptr = nullptr, *csr_col_idx = nullptr, *csr_pos = nullptr;
     int32_t *tile_row = nullptr;  // SpMV tiles of whole block rows
+    int4 *tile_info = nullptr;    // per tile {first row, end row, first block, end block}
     int32_t n_tiles = 0;
 
     // assembled system

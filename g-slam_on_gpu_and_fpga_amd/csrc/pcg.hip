@@ -95,102 +95,176 @@ pcg_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ b
 // A(k): stopping test on the partials of B(k-1), then q = A p with
 // p = z + beta p_old formed per gathered column; p.q partials.
 //
-// One workgroup per tile of whole block rows (<= 256 blocks, <= 85 rows):
-// one lane per 3x3 block (its 9 values, column index and the gathered p
-// entries), the block's 3 products staged in LDS, then one thread per
-// output row (3 per block row) sums its row's products in block order.
-// Measured on the N=100k grid-city pattern: 9.9 us per launch against 12-17
-// us for lane groups per block row (2, 4 or 8 lanes) with or without
-// 16-byte aligned block loads.
-__global__ void __launch_bounds__(kThreads)
+// One workgroup per tile of whole block rows (<= 256 blocks, <= 85 rows;
+// tile_info[t] = {first row, end row, first block, end block}): one lane per
+// 3x3 block (its 9 values, column index and the gathered p entries), the
+// block's 3 products staged in LDS, then one thread per output row (3 per
+// block row) sums its row's products in block order.
+//
+// At N = 100k the matrix and vectors sit in the MI355X Infinity Cache and the
+// kernel is bound by its chain of dependent loads, not by bandwidth: every
+// load that does not need beta (tile record -> block values, column index ->
+// gathered z / p_old, the output row's range and z / p_old) is issued before
+// the partial sums that give beta are loaded and reduced (one barrier).  To
+// hold fewer registers meanwhile, each lane keeps V z_j and V p_old_j (6
+// values instead of 15) and forms V p_j = V z_j + beta V p_old_j afterwards.
+struct SpmvTileRegs {
+    int32_t r0, r1, b0, b1, ob0, ob1;
+    bool has_blk, has_out;
+    double yz[3], yp[3], zo, po;
+};
+
+__device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_info,
+                                      const int32_t *__restrict__ brow_ptr,
+                                      const int32_t *__restrict__ bcol,
+                                      const double *__restrict__ bval,
+                                      const double *__restrict__ z,
+                                      const double *__restrict__ p_old, SpmvTileRegs &T)
+{
+    const int4 ti = tile_info[t];
+    T.r0 = ti.x;
+    T.r1 = ti.y;
+    T.b0 = ti.z;
+    T.b1 = ti.w;
+    const int32_t bb = T.b0 + static_cast<int32_t>(threadIdx.x);
+    T.has_blk = bb < T.b1;
+    const int32_t orow = T.r0 + static_cast<int32_t>(threadIdx.x) / 3;
+    T.has_out = static_cast<int32_t>(threadIdx.x) < 3 * (T.r1 - T.r0);
+    if (T.has_blk) {
+        const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+        const double *v = bval + 9 * static_cast<int64_t>(bb);
+        const double z0 = z[j], z1 = z[j + 1], z2 = z[j + 2];
+        const double q0 = p_old[j], q1 = p_old[j + 1], q2 = p_old[j + 2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T.yz[a] = v[3 * a] * z0 + v[3 * a + 1] * z1 + v[3 * a + 2] * z2;
+            T.yp[a] = v[3 * a] * q0 + v[3 * a + 1] * q1 + v[3 * a + 2] * q2;
+        }
+    }
+    if (T.has_out) {
+        T.ob0 = brow_ptr[orow];
+        T.ob1 = brow_ptr[orow + 1];
+        const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
+        T.zo = z[o];
+        T.po = p_old[o];
+    }
+}
+
+__device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
+                                       const int32_t *__restrict__ bcol,
+                                       const double *__restrict__ bval,
+                                       const double *__restrict__ z,
+                                       const double *__restrict__ p_old,
+                                       double *__restrict__ p_new, double *__restrict__ q,
+                                       double *ys)
+{
+    const int a = static_cast<int>(threadIdx.x) % 3;
+    double acc = 0.0;
+    if (T.has_blk) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ys[r * kTileBlocks + threadIdx.x] = T.yz[r] + beta * T.yp[r];
+    }
+    __syncthreads();
+    if (T.has_out) {
+        const int32_t hi = min(T.ob1, T.b0 + kTileBlocks);
+        for (int32_t c = max(T.ob0, T.b0); c < hi; ++c) acc += ys[a * kTileBlocks + (c - T.b0)];
+    }
+    __syncthreads();
+    // a single block row longer than one tile is walked in further chunks
+    for (int32_t cb = T.b0 + kTileBlocks; cb < T.b1; cb += kTileBlocks) {
+        const int32_t bb = cb + static_cast<int32_t>(threadIdx.x);
+        if (bb < T.b1) {
+            const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+            const double p0 = z[j] + beta * p_old[j];
+            const double p1 = z[j + 1] + beta * p_old[j + 1];
+            const double p2 = z[j + 2] + beta * p_old[j + 2];
+            const double *v = bval + 9 * static_cast<int64_t>(bb);
+            ys[threadIdx.x] = v[0] * p0 + v[1] * p1 + v[2] * p2;
+            ys[kTileBlocks + threadIdx.x] = v[3] * p0 + v[4] * p1 + v[5] * p2;
+            ys[2 * kTileBlocks + threadIdx.x] = v[6] * p0 + v[7] * p1 + v[8] * p2;
+        }
+        __syncthreads();
+        if (T.has_out) {
+            const int32_t lo = max(T.ob0, cb), hi = min(T.ob1, cb + kTileBlocks);
+            for (int32_t c = lo; c < hi; ++c) acc += ys[a * kTileBlocks + (c - cb)];
+        }
+        __syncthreads();
+    }
+    double pq = 0.0;
+    if (T.has_out) {
+        const int64_t o = 3 * static_cast<int64_t>(T.r0 + static_cast<int32_t>(threadIdx.x) / 3) + a;
+        const double pi = T.zo + beta * T.po;
+        p_new[o] = pi;
+        q[o] = acc;
+        pq = pi * acc;
+    }
+    return pq;
+}
+
+// (at most 72 VGPRs: 7 workgroups per CU, 1792 on the chip, keep the 1683
+// tiles of the N = 100k grid city in one round; at 92 VGPRs, 5 per CU, the
+// kernel measured 16 us in a solve.  A 64-VGPR cap spills.)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
-                const double *__restrict__ bval, const int32_t *__restrict__ tile_row,
+                const double *__restrict__ bval, const int4 *__restrict__ tile_info,
                 int32_t n_tiles, const double *__restrict__ z,
                 const double *__restrict__ p_old, double *__restrict__ p_new,
                 double *__restrict__ q, int32_t k, double tol_rel,
                 const double *__restrict__ pr_cur, int32_t np_cur,
-                const double *__restrict__ pr_prev, int32_t np_prev,
-                double *__restrict__ pq_out, PcgStatus *status)
+                double *__restrict__ rz_hist, double *__restrict__ pq_out, PcgStatus *status)
 {
+    __shared__ double ys[3 * kTileBlocks];
+    __shared__ double red[2][kThreads / kWave];
+    SpmvTileRegs T;
+    const int32_t t0 = blockIdx.x;
+    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, bcol, bval, z, p_old, T);
+    // rr_k, rz_k in one reduction (<= kMaxGrid = 2 x kThreads partials per
+    // array: one round of loads); rz_{k-1} as reduced by A(k-1)
+    const double rz_prev = k > 0 ? rz_hist[(k + 1) & 1] : 1.0;
+    double red_v[2];
+    red_v[0] = strided_sum<kThreads, 2>(pr_cur, np_cur, 0.0);
+    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
+    block_allsum<2, kThreads>(red_v, red);
     if (status->done) return;
-    double cur[2];
-    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);  // rr_k, rz_k
+    const double rr = red_v[0], rz = red_v[1];
     double tol;
     if (k == 0) {
-        tol = tol_rel * sqrt(cur[0]);
+        tol = tol_rel * sqrt(rr);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            status->bnorm = sqrt(cur[0]);
+            status->bnorm = sqrt(rr);
             status->tol = tol;
-            status->rnorm = sqrt(cur[0]);
+            status->rnorm = sqrt(rr);
         }
     } else {
         tol = status->tol;
-        if (sqrt(cur[0]) < tol) {  // MatrixOps.c:366: checked after the update
+        if (sqrt(rr) < tol) {  // MatrixOps.c:366: checked after the update
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 status->done = 1;
                 status->converged = 1;
                 status->iters = k;
-                status->rnorm = sqrt(cur[0]);
+                status->rnorm = sqrt(rr);
             }
             return;
         }
     }
-    double beta = 0.0;
-    if (k > 0) {
-        double prev[2];
-        sum_partials<2>(pr_prev, np_prev, kPartStride, prev);
-        beta = cur[1] / prev[1];
-    }
-    __shared__ double ys[3 * kTileBlocks];
+    const double beta = k > 0 ? rz / rz_prev : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) rz_hist[k & 1] = rz;
     double pq[1] = {0.0};
-    for (int32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const int32_t r0 = tile_row[t], r1 = tile_row[t + 1];
-        const int32_t b0 = brow_ptr[r0], b1 = brow_ptr[r1];
-        const int32_t orow = r0 + static_cast<int32_t>(threadIdx.x) / 3;
-        const int a = static_cast<int>(threadIdx.x) % 3;
-        const bool has_out = static_cast<int32_t>(threadIdx.x) < 3 * (r1 - r0);
-        double acc = 0.0;
-        // a single block row longer than one tile is walked in chunks
-        for (int32_t cb = b0; cb < b1; cb += kTileBlocks) {
-            const int32_t bb = cb + static_cast<int32_t>(threadIdx.x);
-            if (bb < b1) {
-                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
-                const double p0 = z[j] + beta * p_old[j];
-                const double p1 = z[j + 1] + beta * p_old[j + 1];
-                const double p2 = z[j + 2] + beta * p_old[j + 2];
-                const double *v = bval + 9 * static_cast<int64_t>(bb);
-                ys[threadIdx.x] = v[0] * p0 + v[1] * p1 + v[2] * p2;
-                ys[kTileBlocks + threadIdx.x] = v[3] * p0 + v[4] * p1 + v[5] * p2;
-                ys[2 * kTileBlocks + threadIdx.x] = v[6] * p0 + v[7] * p1 + v[8] * p2;
-            }
-            __syncthreads();
-            if (has_out) {
-                const int32_t lo = max(brow_ptr[orow], cb);
-                const int32_t hi = min(brow_ptr[orow + 1], cb + kTileBlocks);
-                for (int32_t c = lo; c < hi; ++c) acc += ys[a * kTileBlocks + (c - cb)];
-            }
-            __syncthreads();
-        }
-        if (has_out) {
-            const int64_t o = 3 * static_cast<int64_t>(orow) + a;
-            const double pi = z[o] + beta * p_old[o];
-            p_new[o] = pi;
-            q[o] = acc;
-            pq[0] += pi * acc;
-        }
+    for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
+        if (t != t0) spmv_tile_load(t, tile_info, brow_ptr, bcol, bval, z, p_old, T);
+        pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, ys);
     }
     write_partials<1>(pq, pq_out, kSpmvMaxGrid);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         status->iters = k;
-        status->rnorm = sqrt(cur[0]);
+        status->rnorm = sqrt(rr);
     }
 }
 
 // B(k): alpha from the p.q partials, x/r/z update, r.r / r.z partials.
-// One thread per pose: its 21 loads (x, r, p, q, D^-1) are all in flight at
-// once; D^-1 is stored column-major per entry ([c][3i+a]).  Measured in a
-// PCG solve at N=100k: 9.1 us against 12.7 us for an entry-parallel variant
-// whose grid-stride passes serialise on LDS barriers.
+// One thread per pose: its 21 loads (x, r, p, q, D^-1) are issued before
+// the p.q and r.z partials are loaded and reduced; D^-1 is stored
+// column-major per entry ([c][3i+a]).
 __global__ void __launch_bounds__(kThreads)
 pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
                   const double *__restrict__ dinv, double *__restrict__ x,
@@ -199,34 +273,55 @@ pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
                   const double *__restrict__ pr_cur, int32_t np_cur,
                   double *__restrict__ pr_out, PcgStatus *status)
 {
+    __shared__ double red[2][kThreads / kWave];
+    const int64_t dim = 3 * static_cast<int64_t>(n);
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double xp[3], rp[3], pp[3], qp[3], di[9];
+    auto load = [&](int32_t ii) {
+        const int64_t o = 3 * static_cast<int64_t>(ii);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xp[c] = x[o + c];
+            rp[c] = r[o + c];
+            pp[c] = p[o + c];
+            qp[c] = q[o + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) di[3 * c + a] = dinv[c * dim + o + a];
+    };
+    if (i < n) load(i);
+    double red_v[2];
+    red_v[0] = strided_sum<kThreads>(pq_in, np_pq, 0.0);
+    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
+    block_allsum<2, kThreads>(red_v, red);
     if (status->done) return;
-    double pq[1], cur[2];
-    sum_partials<1>(pq_in, np_pq, kSpmvMaxGrid, pq);
-    if (pq[0] == 0.0) {  // MatrixOps.c:357
+    if (red_v[0] == 0.0) {  // MatrixOps.c:357
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
             status->flags |= 1;
         }
         return;
     }
-    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
-    const double alpha = cur[1] / pq[0];
-    const int64_t dim = 3 * static_cast<int64_t>(n);
+    const double alpha = red_v[1] / red_v[0];
     double v[2] = {0.0, 0.0};
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (; i < n; i += gridDim.x * blockDim.x) {
+        if (i != static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x)) load(i);
         const int64_t o = 3 * static_cast<int64_t>(i);
         double rr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            x[o + c] += alpha * p[o + c];
-            rr[c] = r[o + c] - alpha * q[o + c];
+            x[o + c] = xp[c] + alpha * pp[c];
+            rr[c] = rp[c] - alpha * qp[c];
             r[o + c] = rr[c];
         }
         double zz[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-            zz[a] = dinv[o + a] * rr[0] + dinv[dim + o + a] * rr[1] + dinv[2 * dim + o + a] * rr[2];
-        z[o] = zz[0]; z[o + 1] = zz[1]; z[o + 2] = zz[2];
+        for (int a = 0; a < 3; ++a) zz[a] = di[a] * rr[0] + di[3 + a] * rr[1] + di[6 + a] * rr[2];
+        z[o] = zz[0];
+        z[o + 1] = zz[1];
+        z[o + 2] = zz[2];
         v[0] += rr[0] * rr[0] + rr[1] * rr[1] + rr[2] * rr[2];
         v[1] += rr[0] * zz[0] + rr[1] * zz[1] + rr[2] * zz[2];
     }
@@ -393,9 +488,8 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
         for (int32_t c = 0; c < m; ++c, ++k) {
             double *pold = g.p[k & 1], *pnew = g.p[(k + 1) & 1];
             hipLaunchKernelGGL(pcg_spmv_kernel, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
-                               g.bcol, g.bval, g.tile_row, g.n_tiles, g.z, pold, pnew, g.q, k, o.tol,
-                               PR[k & 1], static_cast<int32_t>(ge), PR[(k + 1) & 1],
-                               static_cast<int32_t>(ge), PQ, st);
+                               g.bcol, g.bval, g.tile_info, g.n_tiles, g.z, pold, pnew, g.q, k, o.tol,
+                               PR[k & 1], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ, st);
             hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, pnew,
                                g.q, g.dinv, x_out, g.r, g.z, n, PQ, static_cast<int32_t>(gt),
                                PR[k & 1], static_cast<int32_t>(ge), PR[(k + 1) & 1], st);
@@ -461,9 +555,9 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
     if (which < GSLAM_KERNEL_SPMV_FP64 || which > GSLAM_KERNEL_ASSEMBLY) return GSLAM_ERR_ARG;
     auto spmv = [&]() {
         hipLaunchKernelGGL(pcg_spmv_kernel, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
-                           g.bcol, g.bval, g.tile_row, g.n_tiles, g.z, g.p[0], g.p[1], g.q, 0,
-                           1e-30, PR[0], static_cast<int32_t>(ge), PR[1], static_cast<int32_t>(ge),
-                           PQ, g.status);
+                           g.bcol, g.bval, g.tile_info, g.n_tiles, g.z, g.p[0], g.p[1], g.q, 0,
+                           1e-30, PR[0], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ,
+                           g.status);
     };
     auto update = [&]() {
         hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.p[1],

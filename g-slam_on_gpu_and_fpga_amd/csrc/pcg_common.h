@@ -22,7 +22,8 @@ constexpr int kPartOffResid = 6 * kPartStride;  // residual / update norms (2 ar
 // beyond), so its p.q partials get an array of their own.
 constexpr int kSpmvMaxGrid = 4096;
 constexpr int kPartOffPQT = 8 * kPartStride;    // p.q of the tiled SpMV
-constexpr int kPartTotal = 8 * kPartStride + kSpmvMaxGrid;
+constexpr int kPartOffRz = 8 * kPartStride + kSpmvMaxGrid;  // reduced r.z by parity (2)
+constexpr int kPartTotal = kPartOffRz + 8;
 
 constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 
@@ -54,6 +55,23 @@ __device__ inline void write_partials(double (&v)[K], double *out, int stride)
     }
 }
 
+// acc += part[i] for i = threadIdx.x, +NT, ... < np, in that order, with
+// the loads of M consecutive terms issued together (predicated, absent terms
+// add 0.0); a plain loop waits for each load before the next add, one memory
+// round trip per term.
+template <int NT, int M = 8>
+__device__ inline double strided_sum(const double *__restrict__ part, int np, double acc)
+{
+    for (int base = threadIdx.x; base < np; base += M * NT) {
+        double t[M];
+#pragma unroll
+        for (int u = 0; u < M; ++u) t[u] = (base + u * NT < np) ? part[base + u * NT] : 0.0;
+#pragma unroll
+        for (int u = 0; u < M; ++u) acc += t[u];
+    }
+    return acc;
+}
+
 // Sum of np partials per quantity in a fixed order; every thread of every
 // workgroup obtains the same bits.
 template <int K, int NT = kThreads>
@@ -63,10 +81,7 @@ __device__ inline void sum_partials(const double *part, int np, int stride, doub
     __shared__ double res[K];
     double v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        v[k] = 0.0;
-        for (int i = threadIdx.x; i < np; i += NT) v[k] += part[k * stride + i];
-    }
+    for (int k = 0; k < K; ++k) v[k] = strided_sum<NT>(part + k * stride, np, 0.0);
     block_sum<K, NT>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
