@@ -5,23 +5,35 @@ One step = one Gauss-Newton step of the hot path on the GPU: edge evaluation
 and Omega/xi assembly, block-Jacobi PCG to the tolerance, pose update.  Every
 timed step starts from the same dead-reckoned initial guess (a 2.4 MB pose
 upload inside the timed loop), so every step solves the same system and the
-accuracy check below applies to each of them.
+accuracy check applies to each of them.
 
-Workload: synthetic grid-city pose graph (gslam.generators.grid_city), N =
-100,000 poses, seed 0, lambda 0.1, angle wrap on; both solvers at relative
-tolerance 1e-12.  Accuracy: relative dx error of each solver against an FP64
-PCG solve of the same system at tolerance 1e-13.
+Headline workload: synthetic grid-city pose graph (gslam.generators.grid_city),
+N = 100,000 poses, seed 0, lambda 0.1, angle wrap on.  Accuracy: relative dx
+error against an FP64 PCG solve of the same system at tolerance 1e-14 (the
+attainable floor: solves at 1e-14 / 3e-14 / 1e-13 agree to 1-6e-9).  The FP64
+solver runs at tolerance 3e-13, the loosest that meets rel dx err <= 1e-8 on
+this graph (1e-12 gives 1.1-1.3e-8); the mixed solver runs at 1e-12, the
+tightest its FP64 true-residual refreshes certify here, and is reported with
+its accuracy.
 
-Also reported: the live per-launch duration of the dominant kernel (the FP64
-BSR SpMV) from HIP events, its algorithmic bytes and the HBM roofline
-fraction, and the CPU oracle (FP64, one thread) timed on a bounded sample of
-the same GN step and extrapolated to the FP64 solve's iteration count.
+Also reported:
+* roofline: the FP64 BSR SpMV's mean per-launch duration inside the timed
+  solves (HIP events around sampled launches on the solver stream), its
+  algorithmic bytes and the HBM fraction; HBM traffic per launch from the
+  committed rocprofv3 counter summary when present;
+* cpu_baseline: the CPU oracle (FP64, one thread) on a bounded sample of the
+  same GN step, extrapolated to the FP64 solve's iteration count;
+* gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) against
+  the plain FP64 Gauss-Jordan and numpy.linalg.solve (FP64), grid-city N=1600;
+* adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
+  central finite differences on sampled edges, optional Adam steps).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,6 +42,9 @@ sys.path.insert(0, os.path.join(ROOT, "g-slam_on_gpu_and_fpga_amd"))
 
 METRIC = "ms per Gauss-Newton step, mixed vs FP64 PCG at N=100k, rel Δx err ≤1e-8"
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK = 157.3   # TFLOP/s, MI355X dense FP32 matrix
+FP64_PEAK = 78.6         # TFLOP/s, MI355X FP64 (vector and matrix)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_spmv_grid100k_traffic.md")
 
 
 def log(msg):
@@ -44,6 +59,253 @@ def spmv_bytes(n_poses, n_blocks):
     return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 4 * 24 * n_poses
 
 
+def measured_traffic():
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def hip_sync():
+    import ctypes
+    ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+
+
+def pcg_section(args, gslam, G, np):
+    t0 = time.time()
+    gc = G.grid_city(args.n, seed=args.seed)
+    log(f"graph N={args.n}: {len(gc['i'])} edges ({time.time() - t0:.1f}s)")
+    init = gc["poses"].copy()
+    g = gslam.Graph(init, **G.graph_kwargs(gc, 0.1, wrap=1))
+    info = g.info
+    log(f"handle: {info}")
+    dx_ref = None
+    if not args.no_accuracy:
+        g.set_poses(init)
+        g.assemble()
+        st = g.solve("pcg64", args.ref_tol, args.max_iter)
+        dx_ref = g.dx()
+        log(f"reference solve (tol {args.ref_tol:g}): {st['iters']} its, "
+            f"true rel res {st['true_rel_res']:.2e}")
+    tols = {"pcg64": args.tol, "mixed": args.mixed_tol}
+    results, samples = {}, []
+    for solver in [s for s in args.solvers.split(",") if s]:
+        tol = tols[solver]
+        acc = {}
+        if dx_ref is not None:
+            g.set_poses(init)
+            g.assemble()
+            st = g.solve(solver, tol, args.max_iter)
+            dx = g.dx()
+            err = float(np.linalg.norm(dx - dx_ref) / np.linalg.norm(dx_ref))
+            acc = dict(rel_dx_err=err, meets_1e8=err <= 1e-8, max_abs_dx_err=float(np.abs(dx - dx_ref).max()),
+                       true_rel_res=st["true_rel_res"], converged=st["converged"],
+                       refreshes=st["reliable_updates"])
+            log(f"{solver}: accuracy {acc}")
+        for _ in range(args.warmup):
+            g.set_poses(init)
+            g.gn_step(solver, tol, args.max_iter)
+        hip_sync()
+        iters = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            g.set_poses(init)
+            st, _ = g.gn_step(solver, tol, args.max_iter,
+                              profile_every=args.profile_every if solver == "pcg64" else 0)
+            iters.append(st["iters"])
+            if solver == "pcg64" and st["spmv_us"] > 0:
+                samples.append((st["spmv_us"], st["update_us"]))
+        hip_sync()
+        ms = 1000.0 * (time.perf_counter() - t_start) / args.steps
+        results[solver] = dict(ms_per_step=ms, tol=tol, pcg_iters=int(np.mean(iters)),
+                               us_per_iter=1000.0 * ms / max(np.mean(iters), 1), **acc)
+        log(f"{solver}: {ms:.1f} ms per GN step, {np.mean(iters):.0f} PCG iterations")
+
+    # dominant kernel: FP64 SpMV, mean duration inside the timed solves
+    byts = spmv_bytes(info["n_poses"], info["n_blocks"])
+    roofline = None
+    if samples:
+        us = float(np.mean([s[0] for s in samples]))
+        achieved = byts / (us * 1e-6) / 1e9
+        roofline = dict(bound="hbm", kernel="pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)",
+                        us_per_launch=us, bytes_per_launch=byts, achieved=achieved,
+                        peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                        traffic=measured_traffic(),
+                        how=f"HIP events around every {args.profile_every}th SpMV launch of "
+                            f"the {args.steps} timed FP64 solves")
+        log(f"spmv64 in solve: {us:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+    g.set_poses(init)
+    g.assemble()
+    kernels = {k: g.time_kernel(k, 200) for k in ("spmv64", "update64", "spmv_mixed", "assembly")}
+    if samples:
+        kernels["spmv64_in_solve"] = float(np.mean([s[0] for s in samples]))
+        kernels["update64_in_solve"] = float(np.mean([s[1] for s in samples]))
+    g.close()
+    return gc, init, info, results, roofline, kernels
+
+
+def cpu_section(args, gc, init, info, results):
+    if args.cpu_iters <= 0 or "pcg64" not in results:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle()
+    rp, ci = O.csr_pattern(info["n_poses"], gc["i"], gc["j"])
+    times = {}
+    for it in (0, args.cpu_iters):
+        p = init.copy()
+        t = time.perf_counter()
+        O.between_gn_step(p, gc["i"], gc["j"], gc["meas"], gc["info"], None, 0.1, rp, ci,
+                          tol=1e-300, max_iter=it, wrap=1)
+        times[it] = time.perf_counter() - t
+    t_iter = (times[args.cpu_iters] - times[0]) / args.cpu_iters
+    full_iters = results["pcg64"]["pcg_iters"]
+    cpu_ms = 1000.0 * (times[0] + t_iter * full_iters)
+    log(f"cpu oracle: {cpu_ms:.0f} ms per GN step (extrapolated)")
+    return dict(value=cpu_ms, unit="ms per GN step", cores=1, kind="port",
+                sample=(f"CPU oracle (oracle/gslam_oracle.c, FP64, 1 thread) GN step on the same "
+                        f"N={info['n_poses']} graph: edges + CSR assembly + pose update timed in "
+                        f"full ({1000 * times[0]:.0f} ms), block-Jacobi PCG timed for "
+                        f"{args.cpu_iters} iterations ({1000 * t_iter:.2f} ms/iteration) and "
+                        f"extrapolated to the GPU FP64 solve's {full_iters} iterations"))
+
+
+def gj_section(args, gslam, G, np):
+    import scipy.sparse as sp
+    out = {}
+    for N in [int(v) for v in args.gj_n.split(",") if v]:
+        gc = G.grid_city(N, seed=0)
+        n = 3 * N
+        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+            g.assemble()
+            rp, ci, va, xi = g.system_csr()
+            A = sp.csr_matrix((va, ci, rp), shape=(n, n)).toarray()
+            t = time.perf_counter()
+            x_np = np.linalg.solve(A, xi)
+            t_np = time.perf_counter() - t
+            del A
+            nx = np.linalg.norm(x_np)
+            row = dict(n=n, numpy_solve_s=t_np)
+            for name, kw in (("ir3", dict(refine_sweeps=3, refine_mode="ir", tol=0.0)),
+                             ("cg", dict(refine_sweeps=200, refine_mode="cg", tol=1e-13))):
+                g.solve("gj_mixed", **kw)   # warm-up (work space allocation)
+                st = g.solve("gj_mixed", **kw)
+                e = float(np.linalg.norm(g.dx() - x_np) / nx)
+                row[f"gj_mixed_{name}"] = dict(ms=st["ms"], factor_ms=st["ms_factor"],
+                                               refine_iters=st["iters"], rel_dx_err_vs_numpy=e,
+                                               factor_tflops=2.0 * n ** 3 / st["ms_factor"] / 1e9)
+                log(f"gj_mixed {name} N={N}: {st['ms']:.2f} ms (factor {st['ms_factor']:.2f}), "
+                    f"{st['iters']} refinement its, err vs numpy {e:.2e}")
+            us = g.time_kernel("gj_trailing", 10)
+            npd = -(-n // 128) * 128
+            fl = 2.0 * (npd - 128) * npd * 128
+            row["trailing_kernel"] = dict(us=us, tflops=fl / us / 1e6, bound="mfma",
+                                          peak=FP32_MFMA_PEAK, frac=fl / us / 1e6 / FP32_MFMA_PEAK)
+            if N <= args.gj64_max_n:
+                st = g.solve("gj64")
+                e = float(np.linalg.norm(g.dx() - x_np) / nx)
+                row["gj64"] = dict(ms=st["ms"], rel_dx_err_vs_numpy=e,
+                                   tflops=2.0 * n ** 3 / st["ms"] / 1e9,
+                                   frac_fp64_peak=2.0 * n ** 3 / st["ms"] / 1e9 / FP64_PEAK)
+                log(f"gj64 N={N}: {st['ms']:.1f} ms, err vs numpy {e:.2e}")
+        out[f"N{N}"] = row
+    return out
+
+
+def adjoint_section(args, gslam, G, np):
+    from gslam import autodiff as AD
+    import torch
+    N = args.adj_n
+    gc = G.grid_city(N, seed=0)
+    conv = dict(solver="pcg64", tol=1e-13, max_iter=400000, diff_tol=1e-13, max_rounds=60)
+    out = dict(n_poses=N, n_edges=int(len(gc["i"])))
+    truth = gc["truth"].reshape(-1, 3)
+
+    def loss_grad(x):
+        p = x.reshape(-1, 3)
+        drift = (p[-1, :2] - p[0, :2]) - (truth[-1, :2] - truth[0, :2])
+        d = p - truth
+        d[:, 2] = (d[:, 2] + np.pi) % (2 * np.pi) - np.pi
+        G_ = 2 * d
+        G_[-1, :2] += 2 * drift
+        G_[0, :2] -= 2 * drift
+        return float(drift @ drift + (d ** 2).sum()), G_.ravel()
+
+    with gslam.Graph(gc["poses"], anchor_prior=1, **G.graph_kwargs(gc, 0.1)) as g:
+        hip_sync()
+        t = time.perf_counter()
+        rounds = g.converge(**conv)
+        hip_sync()
+        out["forward_ms"] = 1000.0 * (time.perf_counter() - t)
+        out["forward_gn_rounds"] = rounds
+        x0 = g.get_poses()
+        L0, dl = loss_grad(x0)
+        t = time.perf_counter()
+        dw, dlam, st = g.adjoint(dl, tol=1e-13)
+        out["adjoint_ms"] = 1000.0 * (time.perf_counter() - t)
+        out["adjoint_pcg_iters"] = st["iters"]
+        out["dl_dlambda"] = dlam
+        log(f"adjoint N={N}: forward {out['forward_ms']:.0f} ms ({rounds} GN rounds), "
+            f"backward {out['adjoint_ms']:.1f} ms ({st['iters']} PCG its)")
+        if args.adj_fd > 0:
+            h = 1e-4
+            ks = np.random.default_rng(0).choice(len(dw), args.adj_fd, replace=False)
+            fd = []
+            for k in ks:
+                v = []
+                for sgn in (1.0, -1.0):
+                    w = np.zeros(len(dw))
+                    w[k] = sgn * h
+                    g.set_log_weights(w)
+                    g.set_poses(x0)
+                    g.converge(**conv)
+                    v.append(loss_grad(g.get_poses())[0])
+                fd.append((v[0] - v[1]) / (2 * h))
+            fd = np.array(fd)
+            g.set_log_weights(None)
+            out["fd_edges"] = int(len(ks))
+            out["fd_rel_err"] = float(np.linalg.norm(dw[ks] - fd) / np.linalg.norm(fd))
+            out["fd_max_rel_err"] = float(np.max(np.abs(dw[ks] - fd) / np.abs(fd).max()))
+            log(f"adjoint vs central FD on {len(ks)} edges: rel {out['fd_rel_err']:.2e}")
+        if args.adam > 0:
+            t = time.perf_counter()
+            _, lam_out, hist = AD.tune(g, gc["poses"], gc["truth"], steps=args.adam, lr=0.05,
+                                       solve_kw=conv)
+            out["adam"] = dict(steps=args.adam, loss_first=hist[0], loss_last=hist[-1],
+                               lambda_last=lam_out, s_per_step=(time.perf_counter() - t) / args.adam)
+            log(f"adam: loss {hist[0]:.4f} -> {hist[-1]:.4f} in {args.adam} steps")
+    return out
+
+
+def profile_kernel(args, gslam, G):
+    """A few launches of one kernel for counter collection."""
+    if args.profile_kernel == "copy":
+        import torch
+        a = torch.ones(16 << 20, dtype=torch.float32, device="cuda")   # 64 MiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy: 5 launches, 64 MiB read + 64 MiB written each")
+        return
+    if args.profile_kernel == "spmv64":
+        gc = G.grid_city(args.n, seed=args.seed)
+        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+            g.assemble()
+            log(f"spmv64: {g.time_kernel('spmv64', 20):.2f} us/launch (23 launches)")
+        return
+    N = int(args.gj_n.split(",")[0] or 1600)
+    gc = G.grid_city(N, seed=0)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        g.solve("gj_mixed", tol=0.0, refine_sweeps=0)
+        log(f"gj_trailing N={N}: {g.time_kernel('gj_trailing', 5):.1f} us/launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,12 +313,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tol", type=float, default=1e-12)
-    ap.add_argument("--max-iter", type=int, default=400000)
+    ap.add_argument("--tol", type=float, default=3e-13, help="FP64 PCG tolerance")
+    ap.add_argument("--mixed-tol", type=float, default=1e-12)
+    ap.add_argument("--ref-tol", type=float, default=1e-14)
+    ap.add_argument("--max-iter", type=int, default=600000)
     ap.add_argument("--solvers", default="mixed,pcg64")
+    ap.add_argument("--profile-every", type=int, default=97,
+                    help="sample every k-th SpMV/update launch of the timed FP64 solves")
     ap.add_argument("--cpu-iters", type=int, default=1500,
                     help="PCG iterations of the bounded CPU oracle sample (0: skip)")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--gj-n", default="1600", help="grid-city sizes of the GJ section ('' skips)")
+    ap.add_argument("--gj64-max-n", type=int, default=2000)
+    ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
+    ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
+    ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
+    ap.add_argument("--profile-kernel", choices=["spmv64", "gj_trailing", "copy"],
+                    help="only launch this kernel a few times and exit (rocprofv3 counter "
+                         "passes; 'copy' is a 64 MiB device copy for calibrating the counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -67,90 +341,13 @@ def main():
     import gslam
     from gslam import generators as G
 
-    t0 = time.time()
-    gc = G.grid_city(args.n, seed=args.seed)
-    log(f"graph N={args.n}: {len(gc['i'])} edges ({time.time() - t0:.1f}s)")
-    init = gc["poses"].copy()
-    g = gslam.Graph(init, **G.graph_kwargs(gc, 0.1, wrap=1))
-    info = g.info
-    log(f"handle: {info}")
-
-    # ---- reference solution of the step's system (FP64 PCG, tol 1e-13)
-    dx_ref = None
-    if not args.no_accuracy:
-        g.set_poses(init)
-        g.assemble()
-        st = g.solve("pcg64", 1e-13, args.max_iter)
-        dx_ref = g.dx()
-        log(f"reference solve: {st['iters']} its, true rel res {st['true_rel_res']:.2e}")
-
-    results = {}
-    for solver in [s for s in args.solvers.split(",") if s]:
-        acc = {}
-        if dx_ref is not None:
-            g.set_poses(init)
-            g.assemble()
-            st = g.solve(solver, args.tol, args.max_iter)
-            dx = g.dx()
-            acc = dict(rel_dx_err=float(np.linalg.norm(dx - dx_ref) / np.linalg.norm(dx_ref)),
-                       true_rel_res=st["true_rel_res"], iters=st["iters"],
-                       converged=st["converged"], refreshes=st["reliable_updates"],
-                       solve_ms=st["ms"])
-            log(f"{solver}: accuracy {acc}")
-        for _ in range(args.warmup):
-            g.set_poses(init)
-            g.gn_step(solver, args.tol, args.max_iter)
-        import ctypes
-        hip = ctypes.CDLL("libamdhip64.so")
-        hip.hipDeviceSynchronize()
-        iters = []
-        t_start = time.perf_counter()
-        for _ in range(args.steps):
-            g.set_poses(init)
-            st, diff = g.gn_step(solver, args.tol, args.max_iter)
-            iters.append(st["iters"])
-        hip.hipDeviceSynchronize()
-        ms = 1000.0 * (time.perf_counter() - t_start) / args.steps
-        results[solver] = dict(ms_per_step=ms, pcg_iters=int(np.mean(iters)),
-                               us_per_iter=1000.0 * ms / max(np.mean(iters), 1), **acc)
-        log(f"{solver}: {ms:.1f} ms per GN step, {np.mean(iters):.0f} PCG iterations")
-
-    # ---- dominant kernel: FP64 SpMV, live per-launch duration (HIP events)
-    g.set_poses(init)
-    g.assemble()
-    us = g.time_kernel("spmv64", 300)
-    byts = spmv_bytes(info["n_poses"], info["n_blocks"])
-    achieved = byts / (us * 1e-6) / 1e9
-    roofline = dict(bound="hbm", kernel="pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)",
-                    us_per_launch=us, bytes_per_launch=byts, achieved=achieved,
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None)
-    log(f"spmv64: {us:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
-    extra = {k: g.time_kernel(k, 100) for k in ("update64", "spmv_mixed", "assembly")}
-
-    # ---- CPU oracle on a bounded sample of the same GN step
-    cpu = None
-    if args.cpu_iters > 0 and "pcg64" in results:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle
-        O = Oracle()
-        rp, ci = O.csr_pattern(info["n_poses"], gc["i"], gc["j"])
-        times = {}
-        for it in (0, args.cpu_iters):
-            p = init.copy()
-            t = time.perf_counter()
-            O.between_gn_step(p, gc["i"], gc["j"], gc["meas"], gc["info"], None, 0.1, rp, ci,
-                              tol=1e-300, max_iter=it, wrap=1)
-            times[it] = time.perf_counter() - t
-        t_iter = (times[args.cpu_iters] - times[0]) / args.cpu_iters
-        full_iters = results["pcg64"]["pcg_iters"]
-        cpu_ms = 1000.0 * (times[0] + t_iter * full_iters)
-        cpu = dict(value=cpu_ms, unit="ms per GN step", cores=1, kind="port",
-                   sample=(f"CPU oracle (oracle/gslam_oracle.c, FP64, 1 thread) GN step on the same "
-                           f"N={info['n_poses']} graph: edges + CSR assembly + pose update timed in "
-                           f"full ({1000 * times[0]:.0f} ms), block-Jacobi PCG timed for "
-                           f"{args.cpu_iters} iterations ({1000 * t_iter:.2f} ms/iteration) and "
-                           f"extrapolated to the GPU FP64 solve's {full_iters} iterations"))
-        log(f"cpu oracle: {cpu_ms:.0f} ms per GN step (extrapolated)")
+    if args.profile_kernel:
+        profile_kernel(args, gslam, G)
+        return
+    gc, init, info, results, roofline, kernels = pcg_section(args, gslam, G, np)
+    cpu = cpu_section(args, gc, init, info, results)
+    gj = gj_section(args, gslam, G, np) if args.gj_n else None
+    adj = adjoint_section(args, gslam, G, np) if args.adj_n > 0 else None
 
     fp = results.get("pcg64", {})
     mx = results.get("mixed", {})
@@ -170,18 +367,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"grid-city pose graph, N={info['n_poses']} poses, seed {args.seed}, "
                                f"lambda 0.1, one GN step from the dead-reckoned initial guess, "
-                               f"block-Jacobi PCG to rel tol {args.tol:g}",
+                               f"block-Jacobi PCG to rel tol {args.tol:g} (FP64)",
                    "n_poses": info["n_poses"], "n_edges": info["n_edges"],
                    "n_blocks": info["n_blocks"], "nnz": info["nnz"],
                    "headline_solver": "pcg64" if fp else "mixed"},
         "fp64": fp,
         "mixed": mx,
         "mixed_speedup_vs_fp64": (fp["ms_per_step"] / mx["ms_per_step"]) if fp and mx else None,
-        "kernels_us": dict(spmv64=us, **extra),
+        "kernels_us": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "gj": gj,
+        "adjoint": adj,
     }
-    g.close()
     print(json.dumps(out), flush=True)
 
 

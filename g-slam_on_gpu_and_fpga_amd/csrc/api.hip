@@ -29,6 +29,7 @@ DeviceGraph::~DeviceGraph()
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev2) (void)hipEventDestroy(ev2);
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     dense.release();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -178,6 +179,7 @@ void gslam_default_opts(gslam_solve_opts *o)
     o->refresh_every = 64;
     o->check_every = 0;
     o->refine_mode = GSLAM_REFINE_IR;
+    o->profile_every = 0;
 }
 
 void gslam_default_desc(gslam_graph_desc *d)

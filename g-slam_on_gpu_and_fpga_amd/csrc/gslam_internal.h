@@ -114,6 +114,7 @@ struct DeviceGraph {
     float *frec = nullptr;
 
     DenseWork dense;             // GJ_MIXED work space (allocated on first use)
+    std::vector<hipEvent_t> prof_ev;  // in-solve kernel timing (created on first use)
 
     std::vector<void *> allocs;
 

@@ -17,6 +17,8 @@
 // atomic counter costs ~15-35 us per iteration on MI355X at 1-2k
 // workgroups, the redundant re-read of <= 512 partials ~1 us.  Only
 // workgroup 0 writes the status record the host polls.
+#include <hip/hip_ext.h>
+
 #include "device_common.h"
 #include "pcg_common.h"
 
@@ -484,15 +486,46 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
                        g.p[0], g.p[1], PR[0], st);
     GSLAM_HIP(hipGetLastError());
     int32_t k = 0;
+    // optional in-solve timing: events around the SpMV and the update of
+    // every profile_every-th iteration (4 events per sample)
+    constexpr int kMaxSamples = 128;
+    int n_samp = 0;
+    if (o.profile_every > 0 && g.prof_ev.empty()) {
+        g.prof_ev.resize(4 * kMaxSamples, nullptr);
+        for (hipEvent_t &e : g.prof_ev) GSLAM_HIP(hipEventCreate(&e));
+    }
     auto enqueue_iters = [&](int32_t m) {
         for (int32_t c = 0; c < m; ++c, ++k) {
             double *pold = g.p[k & 1], *pnew = g.p[(k + 1) & 1];
-            hipLaunchKernelGGL(pcg_spmv_kernel, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
-                               g.bcol, g.bval, g.tile_info, g.n_tiles, g.z, pold, pnew, g.q, k, o.tol,
-                               PR[k & 1], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ, st);
-            hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, pnew,
-                               g.q, g.dinv, x_out, g.r, g.z, n, PQ, static_cast<int32_t>(gt),
-                               PR[k & 1], static_cast<int32_t>(ge), PR[(k + 1) & 1], st);
+            const bool samp = o.profile_every > 0 && k % o.profile_every == 0 && n_samp < kMaxSamples;
+            hipEvent_t *ev = samp ? &g.prof_ev[4 * n_samp++] : nullptr;
+            auto spmv_args = [&](auto launch) {
+                launch(pcg_spmv_kernel, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.n_tiles, g.z,
+                       pold, pnew, g.q, k, o.tol, PR[k & 1], static_cast<int32_t>(ge),
+                       g.part + kPartOffRz, PQ, st);
+            };
+            auto update_args = [&](auto launch) {
+                launch(pcg_update_kernel, pnew, g.q, g.dinv, x_out, g.r, g.z, n, PQ,
+                       static_cast<int32_t>(gt), PR[k & 1], static_cast<int32_t>(ge),
+                       PR[(k + 1) & 1], st);
+            };
+            const dim3 blk(kThreads);
+            if (ev) {
+                // the events are tied to the kernel's own dispatch packet
+                spmv_args([&](auto kern, auto... a) {
+                    hipExtLaunchKernelGGL(kern, dim3(gt), blk, 0, g.stream, ev[0], ev[1], 0, a...);
+                });
+                update_args([&](auto kern, auto... a) {
+                    hipExtLaunchKernelGGL(kern, dim3(ge), blk, 0, g.stream, ev[2], ev[3], 0, a...);
+                });
+            } else {
+                spmv_args([&](auto kern, auto... a) {
+                    hipLaunchKernelGGL(kern, dim3(gt), blk, 0, g.stream, a...);
+                });
+                update_args([&](auto kern, auto... a) {
+                    hipLaunchKernelGGL(kern, dim3(ge), blk, 0, g.stream, a...);
+                });
+            }
         }
         hipLaunchKernelGGL(pcg_finalize_kernel, dim3(1), dim3(kThreads), 0, g.stream, PR[k & 1],
                            static_cast<int32_t>(ge), k, o.max_iter, st);
@@ -510,6 +543,21 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
     s.reliable_updates = 0;
     s.rel_res = ps.bnorm > 0 ? ps.rnorm / ps.bnorm : ps.rnorm;
     s.ms = ms;
+    s.spmv_us = s.update_us = 0.0;
+    if (n_samp > 0) {
+        // samples past the converged iteration timed early-exit launches
+        const int use = std::max(1, std::min(n_samp, (ps.iters + o.profile_every - 1) / o.profile_every));
+        double a = 0.0, u = 0.0;
+        for (int i = 0; i < use; ++i) {
+            float t0 = 0.f, t1 = 0.f;
+            GSLAM_HIP(hipEventElapsedTime(&t0, g.prof_ev[4 * i], g.prof_ev[4 * i + 1]));
+            GSLAM_HIP(hipEventElapsedTime(&t1, g.prof_ev[4 * i + 2], g.prof_ev[4 * i + 3]));
+            a += t0;
+            u += t1;
+        }
+        s.spmv_us = 1000.0 * a / use;
+        s.update_us = 1000.0 * u / use;
+    }
     GSLAM_TRY(true_rel_residual(g, b, x_out, &s.true_rel_res));
     return GSLAM_OK;
 }

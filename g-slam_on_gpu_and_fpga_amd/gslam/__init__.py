@@ -55,13 +55,15 @@ class _Info(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("refine_sweeps", C.c_int32), ("refresh_every", C.c_int32),
-                ("check_every", C.c_int32), ("refine_mode", C.c_int32)]
+                ("check_every", C.c_int32), ("refine_mode", C.c_int32),
+                ("profile_every", C.c_int32)]
 
 
 class _Stats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("reliable_updates", C.c_int32),
                 ("flags", C.c_int32), ("rel_res", C.c_double), ("true_rel_res", C.c_double),
-                ("ms", C.c_double), ("ms_factor", C.c_double)]
+                ("ms", C.c_double), ("ms_factor", C.c_double), ("spmv_us", C.c_double),
+                ("update_us", C.c_double)]
 
 
 _LIB = None
@@ -166,7 +168,7 @@ def read_log(path: str) -> dict:
 
 
 def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
-              check_every=0, refine_mode="ir") -> _Opts:
+              check_every=0, refine_mode="ir", profile_every=0) -> _Opts:
     o = _Opts()
     lib().gslam_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -176,13 +178,14 @@ def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_e
     o.refresh_every = refresh_every
     o.check_every = check_every
     o.refine_mode = REFINE[refine_mode] if isinstance(refine_mode, str) else int(refine_mode)
+    o.profile_every = profile_every
     return o
 
 
 def _stats(s: _Stats) -> dict:
     return dict(iters=s.iters, converged=bool(s.converged), reliable_updates=s.reliable_updates,
                 flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms,
-                ms_factor=s.ms_factor)
+                ms_factor=s.ms_factor, spmv_us=s.spmv_us, update_us=s.update_us)
 
 
 class Graph:
@@ -338,12 +341,22 @@ class Graph:
         return dw, dlam.value, _stats(s)
 
     def converge(self, solver="pcg64", tol=1e-12, max_iter=200000, diff_tol=1e-12,
-                 max_rounds=50, **kw) -> int:
-        """Gauss-Newton steps until ||dx||/sqrt(3N) < diff_tol; returns rounds."""
+                 max_rounds=50, patience=3, **kw) -> int:
+        """Gauss-Newton steps until ||dx||/sqrt(3N) < diff_tol, or until the
+        diff has not improved for `patience` rounds (the rounding floor of the
+        step, as the reference's 20-round no-improvement stop,
+        src/cpu_pcg_fp64/GSLAM.c:119-130); returns the rounds run."""
+        best, stale = float("inf"), 0
         for r in range(1, max_rounds + 1):
             _, diff = self.gn_step(solver, tol, max_iter, **kw)
             if diff < diff_tol:
                 return r
+            if diff < best:
+                best, stale = diff, 0
+            else:
+                stale += 1
+                if stale >= patience:
+                    return r
         return max_rounds
 
     def dx(self) -> np.ndarray:

@@ -112,6 +112,9 @@ typedef struct {
                                    convergence test every k iterations (64)  */
     int32_t check_every;        /* PCG: host polls convergence every k iters */
     int32_t refine_mode;        /* GJ_MIXED: GSLAM_REFINE_IR or _CG          */
+    int32_t profile_every;      /* PCG_FP64: bracket the SpMV and update
+                                   launches of every k-th iteration (at most
+                                   128) with HIP events; 0 = off             */
 } gslam_solve_opts;
 
 /* GJ_MIXED refinement on the true residual xi - Omega dx (FP64), with the
@@ -135,6 +138,8 @@ typedef struct {
     double true_rel_res;        /* ||xi - Omega dx|| / ||xi||, FP64          */
     double ms;                  /* device time of the solve (HIP events)     */
     double ms_factor;           /* GJ: device time of the dense inversion     */
+    double spmv_us, update_us;  /* PCG_FP64 with profile_every: mean sampled
+                                   per-launch durations inside the solve (us) */
 } gslam_solve_stats;
 
 /* Fills opts with the reference defaults (PCG FP64, tol 1e-6, 500 iterations:
