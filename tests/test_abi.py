@@ -35,14 +35,17 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_defaults_reproduce_reference_constants():
-    o = gslam.make_opts()
+    o = gslam._Opts()
+    gslam.lib().gslam_default_opts(C.byref(o))
     assert (o.solver, o.tol, o.max_iter) == (0, 1e-6, 500)  # MatrixOps.c:345-347
+    assert (o.refine_sweeps, o.refine_mode, o.profile_every) == (3, 0, 0)
+    assert (o.refresh_every, o.check_every) == (64, 0)
     d = gslam._Desc()
     gslam.lib().gslam_default_desc(C.byref(d))
     assert list(d.mns) == [0.19, 0.001, 0.13, 0.2]           # GSLAM.c:66
     assert list(d.snr) == [0.14, 0.05]                        # GSLAM.c:301
     assert d.anchor == 1e6                                    # MatrixOps.c:227
-    assert d.wrap == 0 and d.pair_order == 0
+    assert d.wrap == 0 and d.pair_order == 0 and d.anchor_prior == 0
 
 
 def _gpu_present():
@@ -82,3 +85,31 @@ def test_read_log_matches_oracle_reader(oracle):
         for k in ("u_nu", "u_omega", "poses", "z_range", "z_bearing", "z_step", "z_lm"):
             assert np.array_equal(a[k], b[k]), (f, k)
         assert a["delta"] == b["delta"]
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Field offsets and sizes of the binding's ctypes mirrors equal the C
+    header's (compiled with gcc), so a field added on one side only fails."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    pairs = [("gslam_solve_opts", gslam._Opts), ("gslam_solve_stats", gslam._Stats),
+             ("gslam_graph_desc", gslam._Desc), ("gslam_graph_info", gslam._Info)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gslam.h"', "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            cf = "lambda" if fname == "lambda_" else fname
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for cname, cls in pairs:
+        assert int(out[f"{cname} size"]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(out[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)

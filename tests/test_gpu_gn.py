@@ -62,6 +62,34 @@ def test_gn_converged_between_graph_against_oracle(oracle, wrap):
             assert np.abs(dp).max() <= 1e-6, step
 
 
+def test_gn_fp64_pcg_grid10k_against_cpu_oracle(oracle):
+    """FP64 PCG GN run on the N=10,000 grid city (lambda 0.1, PCG tol 1e-10,
+    the reference stop rule diff < 0.01 within 20 rounds, GSLAM.c:119-130):
+    poses within 1e-9 relative of the CPU FP64 PCG oracle run."""
+    gc = G.grid_city(10000, seed=0)
+    n = gc["n_poses"]
+    rp, ci = oracle.csr_pattern(n, gc["i"], gc["j"])
+    p_o = gc["poses"].copy()
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
+        for rounds in range(1, 21):
+            st, diff = g.gn_step("pcg64", 1e-10, 200000)
+            _, it_o, diff_o = oracle.between_gn_step(p_o, gc["i"], gc["j"], gc["meas"], gc["info"],
+                                                     None, 0.1, rp, ci, tol=1e-10, wrap=1)
+            assert st["converged"]
+            # both solves meet 1e-10; a late, small dx carries ~1e-5 relative
+            # error at kappa ~ 1e10 (3.9e-6 measured), enough for the stop rule
+            assert abs(diff - diff_o) <= 1e-4 * diff_o
+            if diff < 0.01:
+                break
+        p = g.get_poses()
+    dp = (p - p_o).reshape(-1, 3)
+    dp[:, 2] = (dp[:, 2] + np.pi) % (2 * np.pi) - np.pi
+    rel = np.linalg.norm(dp) / np.linalg.norm(p_o)
+    print(f"grid10k: {rounds} GN rounds, last diff {diff:.3e}, pose rel diff {rel:.2e}")
+    assert diff_o < 0.01 and rounds < 20
+    assert rel <= 1e-9
+
+
 def test_gn_mixed_matches_fp64_on_park():
     gc = G.park(1500, seed=12)
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 1.0)) as a, \
