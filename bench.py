@@ -292,11 +292,32 @@ def profile_kernel(args, gslam, G):
         torch.cuda.synchronize()
         log("copy: 5 launches, 64 MiB read + 64 MiB written each")
         return
+    if args.profile_kernel == "stream":
+        import torch
+        a = torch.ones(1 << 29, dtype=torch.float32, device="cuda")    # 2 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1000.0 * e0.elapsed_time(e1) / 10
+        gbs = 2 * (1 << 31) / (us * 1e-6) / 1e9
+        log(f"stream: 2 GiB copy {us:.0f} us, {gbs:.0f} GB/s read+write = "
+            f"{100 * gbs / HBM_PEAK_GBS:.1f}% of the 8 TB/s HBM peak")
+        return
     if args.profile_kernel == "spmv64":
         gc = G.grid_city(args.n, seed=args.seed)
         with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
             g.assemble()
-            log(f"spmv64: {g.time_kernel('spmv64', 20):.2f} us/launch (23 launches)")
+            us = g.time_kernel("spmv64", 20)
+            byts = spmv_bytes(g.info["n_poses"], g.info["n_blocks"])
+            gbs = byts / (us * 1e-6) / 1e9
+            log(f"spmv64 N={args.n}: {us:.2f} us/launch (23 launches), {byts / 1e6:.1f} MB "
+                f"algorithmic, {gbs:.0f} GB/s = {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
         return
     N = int(args.gj_n.split(",")[0] or 1600)
     gc = G.grid_city(N, seed=0)
@@ -328,9 +349,10 @@ def main():
     ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
     ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
-    ap.add_argument("--profile-kernel", choices=["spmv64", "gj_trailing", "copy"],
+    ap.add_argument("--profile-kernel", choices=["spmv64", "gj_trailing", "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "
-                         "passes; 'copy' is a 64 MiB device copy for calibrating the counters)")
+                         "passes; 'copy' is a 64 MiB device copy for calibrating the counters, "
+                         "'stream' times a 2 GiB device copy: the achievable HBM bandwidth)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "

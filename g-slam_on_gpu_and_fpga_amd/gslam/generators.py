@@ -53,30 +53,45 @@ def _dead_reckon(z_odo: np.ndarray, p0: np.ndarray) -> np.ndarray:
     return out
 
 
-def _loop_pairs(xy: np.ndarray, radius: float, gap: int):
-    """All (i, j), i < j - gap, with |xy_i - xy_j| < radius (cell hashing)."""
+def _loop_pairs(xy: np.ndarray, radius: float, gap: int, chunk_pairs: int = 1 << 24):
+    """All (i, j), i < j - gap, with |xy_i - xy_j| < radius (cell hashing).
+
+    Pairs come out grouped by neighbour-cell offset (9 offsets in a fixed
+    order), then by i ascending, then in the cell's sorted order -- the order
+    the seeded subsampling of the generators depends on.  Vectorised: each
+    pose's candidate range in every neighbour cell comes from searchsorted
+    over the sorted cell keys, expanded in chunks of at most chunk_pairs."""
+    n = len(xy)
     cell = np.floor(xy / radius).astype(np.int64)
     key = (cell[:, 0] + (1 << 20)) * (1 << 21) + (cell[:, 1] + (1 << 20))
     order = np.argsort(key, kind="stable")
     ks = key[order]
-    uniq, start = np.unique(ks, return_index=True)
-    end = np.append(start[1:], len(ks))
-    lookup = dict(zip(uniq.tolist(), zip(start.tolist(), end.tolist())))
     I, J = [], []
+    idx = np.arange(n, dtype=np.int64)
     for dcx in (-1, 0, 1):
         for dcy in (-1, 0, 1):
             nk = key + dcx * (1 << 21) + dcy
-            for i in range(len(xy)):
-                se = lookup.get(int(nk[i]))
-                if se is None:
-                    continue
-                cand = order[se[0]:se[1]]
-                cand = cand[cand > i + gap]
-                if cand.size:
-                    d = np.hypot(xy[cand, 0] - xy[i, 0], xy[cand, 1] - xy[i, 1])
-                    cand = cand[d < radius]
-                    I.append(np.full(cand.size, i))
-                    J.append(cand)
+            lo = np.searchsorted(ks, nk, side="left")
+            cnt = np.searchsorted(ks, nk, side="right") - lo
+            csum = np.concatenate([[0], np.cumsum(cnt)])
+            a = 0
+            while a < n:
+                # poses [a, b) with at most chunk_pairs candidates (at least one pose)
+                b = int(np.searchsorted(csum, csum[a] + chunk_pairs, side="right")) - 1
+                b = min(max(b, a + 1), n)
+                c = cnt[a:b]
+                tot = int(c.sum())
+                if tot:
+                    ii = np.repeat(idx[a:b], c)
+                    first = np.repeat(lo[a:b] - (csum[a:b] - csum[a]), c)
+                    cand = order[first + np.arange(tot)]
+                    m = cand > ii + gap
+                    ii, cand = ii[m], cand[m]
+                    d = np.hypot(xy[cand, 0] - xy[ii, 0], xy[cand, 1] - xy[ii, 1])
+                    m = d < radius
+                    I.append(ii[m])
+                    J.append(cand[m])
+                a = b
     if not I:
         return np.zeros(0, np.int64), np.zeros(0, np.int64)
     return np.concatenate(I), np.concatenate(J)
