@@ -16,6 +16,8 @@ import os
 
 import numpy as np
 
+from .landmarks import landmark_means  # noqa: F401  (host post-processing, GSLAM.c:141-196)
+
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GSLAM_LIB", os.path.join(PKG_DIR, "libgslam.so"))
 

@@ -15,8 +15,8 @@
  *   orc_log_count/read, orc_obs_pairs, orc_motion_edge, orc_obs_edge,
  *   orc_csr_pattern/values, orc_pcg, orc_block_jacobi, orc_spmv,
  *   orc_gj_inv, orc_gj_inv_f32, orc_dense_values, orc_dense_matvec,
- *   orc_log_gn_run, orc_diff, orc_pose_update: pinned bitwise to the
- *   reference build and fixtures.
+ *   orc_log_gn_run, orc_diff, orc_pose_update, orc_landmark_means: pinned
+ *   bitwise to the reference build and fixtures.
  *   orc_between_edge / orc_between_edges / orc_between_gn_step: no reference
  *   counterpart; pinned by central finite differences of the residual
  *   (J), J^T L J structure, and a direct sparse solve (tests).
@@ -845,4 +845,58 @@ int32_t orc_between_gn_step(int32_t n_poses, double *poses, int64_t n_edges,
     free(ev); free(values); free(xi);
     if (!dx_out) free(dx);
     return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Landmark estimates after the GN loop, src/cpu_pcg_fp64/GSLAM.c:141-196:
+ * landmarks in first-appearance order (make_edges, GSLAM.c:222-285), each
+ * observation gives m = (x + r cos(th + phi), y + r sin(th + phi))
+ * (MapEdge_create, MapEdge.c:12-29: m1 + m2), the estimate is the mean of a
+ * landmark's m in observation order (vec_mean, MatrixOps.c:161-171).  The
+ * reference computes these and frees them without output; its sentinel and
+ * 1024-entry id array limits are not reproduced.  Returns the number of
+ * landmarks; ids / means (2 per landmark) may be NULL to count.            */
+int32_t orc_landmark_means(int32_t n_obs, const double *poses, const int32_t *z_step,
+                           const int32_t *z_lm, const double *z_range,
+                           const double *z_bearing, int32_t *ids_out, double *means_out)
+{
+    if (n_obs <= 0) return 0;
+    int32_t *sorted = (int32_t *)malloc(sizeof(int32_t) * (size_t)n_obs);
+    memcpy(sorted, z_lm, sizeof(int32_t) * (size_t)n_obs);
+    qsort(sorted, (size_t)n_obs, sizeof(int32_t), cmp_i32);
+    int32_t nd = 0;
+    for (int32_t i = 0; i < n_obs; i++)
+        if (i == 0 || sorted[i] != sorted[nd - 1]) sorted[nd++] = sorted[i];
+    int32_t *rank = (int32_t *)malloc(sizeof(int32_t) * (size_t)nd);  /* first-appearance rank */
+    for (int32_t i = 0; i < nd; i++) rank[i] = -1;
+    int32_t n_ids = 0;
+    int32_t *slot = (int32_t *)malloc(sizeof(int32_t) * (size_t)n_obs);
+    for (int32_t i = 0; i < n_obs; i++) {
+        int32_t *p = (int32_t *)bsearch(&z_lm[i], sorted, (size_t)nd, sizeof(int32_t), cmp_i32);
+        size_t pos = (size_t)(p - sorted);
+        if (rank[pos] < 0) rank[pos] = n_ids++;
+        slot[i] = rank[pos];
+    }
+    if (ids_out && means_out) {
+        double *sum = (double *)calloc(2 * (size_t)n_ids, sizeof(double));
+        int32_t *cnt = (int32_t *)calloc((size_t)n_ids, sizeof(int32_t));
+        for (int32_t i = 0; i < n_obs; i++) {   /* file order within each landmark */
+            const double *x = poses + 3 * (int64_t)z_step[i];
+            const double m0 = x[0] + z_range[i] * cos(x[2] + z_bearing[i]);
+            const double m1 = x[1] + z_range[i] * sin(x[2] + z_bearing[i]);
+            sum[2 * slot[i]] += m0;
+            sum[2 * slot[i] + 1] += m1;
+            if (cnt[slot[i]]++ == 0) ids_out[slot[i]] = z_lm[i];
+        }
+        for (int32_t b = 0; b < n_ids; b++) {
+            means_out[2 * b] = sum[2 * b] / cnt[b];
+            means_out[2 * b + 1] = sum[2 * b + 1] / cnt[b];
+        }
+        free(sum);
+        free(cnt);
+    }
+    free(sorted);
+    free(rank);
+    free(slot);
+    return n_ids;
 }

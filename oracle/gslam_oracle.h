@@ -129,6 +129,13 @@ int64_t orc_log_edges(int32_t n_poses, const double *poses,
 /* ||v||_2 / sqrt(dim): src/cpu_pcg_fp64/MatrixOps.c:4-16 + GSLAM.c:104-105 */
 double orc_diff(int32_t dim, const double *dx);
 
+/* Landmark estimates after the GN loop (GSLAM.c:141-196, MapEdge.c:7-69):
+ * first-appearance landmark order, mean of x + r (cos, sin)(th + phi) over a
+ * landmark's observations.  Returns the number of landmarks. */
+int32_t orc_landmark_means(int32_t n_obs, const double *poses, const int32_t *z_step,
+                           const int32_t *z_lm, const double *z_range,
+                           const double *z_bearing, int32_t *ids_out, double *means_out);
+
 /* Pose update hat_x += dx (GSLAM.c:98-102); wrap != 0 wraps theta to
  * (-pi, pi] afterwards (the build's optional angle-wrap mode). */
 void orc_pose_update(int32_t n_poses, double *poses, const double *dx,

@@ -53,6 +53,8 @@ class Oracle:
         L.orc_log_read.argtypes = [C.c_char_p] + [vp] * 10
         L.orc_obs_pairs.argtypes = [C.c_int32, vp, C.c_int32, vp, vp]
         L.orc_obs_pairs.restype = C.c_int64
+        L.orc_landmark_means.argtypes = [C.c_int32] + [vp] * 7
+        L.orc_landmark_means.restype = C.c_int32
         L.orc_motion_edge.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double,
                                       C.c_double, vp, vp, vp]
         L.orc_obs_edge.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double,
@@ -254,6 +256,19 @@ class Oracle:
         self.L.orc_dense_matvec(a.shape[0], _ptr(a), _ptr(x), _ptr(y))
         return y
 
+    def landmark_means(self, g: dict, poses=None):
+        """(ids, means[L, 2]) of the landmarks, GSLAM.c:141-196."""
+        poses = np.ascontiguousarray(g["poses"] if poses is None else poses, np.float64)
+        zs, zl = (np.ascontiguousarray(g[k], np.int32) for k in ("z_step", "z_lm"))
+        zr, zb = (np.ascontiguousarray(g[k], np.float64) for k in ("z_range", "z_bearing"))
+        n = self.L.orc_landmark_means(len(zs), _ptr(poses), _ptr(zs), _ptr(zl), _ptr(zr), _ptr(zb),
+                                      None, None)
+        ids = np.zeros(n, np.int32)
+        means = np.zeros((n, 2))
+        self.L.orc_landmark_means(len(zs), _ptr(poses), _ptr(zs), _ptr(zl), _ptr(zr), _ptr(zb),
+                                  _ptr(ids), _ptr(means))
+        return ids, means
+
     def diff(self, dx):
         dx = np.ascontiguousarray(dx, np.float64)
         return self.L.orc_diff(len(dx), _ptr(dx))
@@ -311,6 +326,17 @@ class _Edge(C.Structure):
                 ("lam", C.c_double), ("hat_e", C.c_double * 2)]
 
 
+class _MapEdge(C.Structure):
+    """struct MapEdge, src/cpu_pcg_fp64/MapEdge.h:7-11."""
+    _fields_ = [("x", _HAT_X), ("z", _Z), ("m", C.c_double * 2), ("Omega", C.c_double * 4),
+                ("xi", C.c_double * 2)]
+
+
+class _LandmarkKeysZ(C.Structure):
+    """struct LandmarkKeysZList, src/cpu_pcg_fp64/LandmarkKeysZList.h:4-7."""
+    _fields_ = [("landmark_id", C.c_uint), ("z", _Z)]
+
+
 class _CSR(C.Structure):
     """CSRMatrix, src/cpu_pcg_fp64/MatrixOps.h:21-27."""
     _fields_ = [("row_ptr", C.POINTER(C.c_int)), ("col_idx", C.POINTER(C.c_int)),
@@ -343,6 +369,10 @@ class RefPCG64:
         L.make_edges.argtypes = [C.POINTER(_HAT_X), C.c_uint, C.POINTER(_Z), C.c_uint,
                                  C.POINTER(_Edge), C.POINTER(C.c_uint), C.c_void_p,
                                  C.POINTER(C.c_uint)]
+        L.MapEdge_create.argtypes = [C.POINTER(_MapEdge), _Z, _Z, C.POINTER(_HAT_X),
+                                     C.POINTER(C.c_double)]
+        L.MapEdge_create.restype = C.c_void_p
+        L.vec_mean.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint, C.c_uint]
 
     @staticmethod
     def _poses(poses):
@@ -397,6 +427,42 @@ class RefPCG64:
         nl = C.c_uint(0)
         self.L.make_edges(hx, len(poses) // 3, zl, nz, edges, C.byref(ne), lkz, C.byref(nl))
         return edges, ne.value, hx
+
+    def landmark_means(self, g: dict, poses=None):
+        """The reference's post-loop (GSLAM.c:141-196) through its own
+        make_edges, MapEdge_create and vec_mean: (ids, means[L, 2])."""
+        poses = g["poses"] if poses is None else poses
+        hx = self._poses(poses)
+        nz = len(g["z_step"])
+        zl = (_Z * max(nz, 1))()
+        for i in range(nz):
+            zl[i].step = int(g["z_step"][i])
+            zl[i].landmark_id = int(g["z_lm"][i])
+            zl[i].z[0] = g["z_range"][i]
+            zl[i].z[1] = g["z_bearing"][i]
+        cap = int(len(self._pairs_count(g["z_lm"]))) + 1
+        edges = (_Edge * cap)()
+        lkz = (_LandmarkKeysZ * max(nz, 1))()
+        ne, nl = C.c_uint(0), C.c_uint(0)
+        self.L.make_edges(hx, len(poses) // 3, zl, nz, edges, C.byref(ne), C.cast(lkz, C.c_void_p),
+                          C.byref(nl))
+        snr = (C.c_double * 2)(0.14, 0.05)
+        ids, means, i = [], [], 0
+        while i < nl.value:
+            j = i
+            while j < nl.value and lkz[j].landmark_id == lkz[i].landmark_id:
+                j += 1
+            m = (C.c_double * (2 * (j - i)))()
+            for k in range(i, j):
+                me = _MapEdge()
+                self.L.MapEdge_create(C.byref(me), lkz[k].z, lkz[i].z, hx, snr)
+                m[2 * (k - i)], m[2 * (k - i) + 1] = me.m[0], me.m[1]
+            mean = (C.c_double * 2)(0.0, 0.0)
+            self.L.vec_mean(mean, m, 2, j - i)
+            ids.append(lkz[i].landmark_id)
+            means.append((mean[0], mean[1]))
+            i = j
+        return np.array(ids, np.int32), np.array(means).reshape(-1, 2)
 
     @staticmethod
     def _pairs_count(z_lm):
