@@ -32,6 +32,14 @@ namespace gslam {
 
 constexpr int kLanesMixed = 4;
 
+// One group of kLanesMixed lanes per pose row, one pass (up to kSpmvMaxGrid
+// workgroups, grid-stride beyond); p.q partials in the kSpmvMaxGrid array.
+static inline unsigned int mixed_spmv_grid(int32_t n)
+{
+    int64_t g = (static_cast<int64_t>(n) * kLanesMixed + kThreads - 1) / kThreads;
+    return static_cast<unsigned int>(std::max<int64_t>(1, std::min<int64_t>(g, kSpmvMaxGrid)));
+}
+
 __global__ void __launch_bounds__(kThreads)
 mixed_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ bval,
                    const double *__restrict__ b, int32_t n, int32_t max_iter,
@@ -155,6 +163,12 @@ __device__ inline void edge_apply(int type, int side, const float4 f0, const flo
 
 // A(k): stopping test (true residual slots only), beta, q = Omega_32 p with
 // p = z + beta p_old on the fly, p.q partials.
+//
+// Latency-bound at the sizes of interest: the row's own z / p_old and, per
+// lane, its first incidence entry (index, other pose, the two factor
+// records, the other pose's z / p_old) are loaded before the r.r / r.z
+// partials are reduced (one barrier); further entries of long rows are read
+// after.  The arithmetic is the same as forming each p on the fly.
 template <int G>
 __global__ void __launch_bounds__(kThreads)
 mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ xlist,
@@ -166,57 +180,98 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
                   const double *__restrict__ pr_prev, int32_t np_prev,
                   double *__restrict__ pq_out, PcgStatus *status)
 {
+    __shared__ double red[3][kThreads / kWave];
+    constexpr int kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    const int64_t row_first = static_cast<int64_t>(blockIdx.x) * kRowsPerTile + threadIdx.x / G;
+    // loads of the first row handled by this thread (independent of beta)
+    float zi[3] = {0.f, 0.f, 0.f}, qi[3] = {0.f, 0.f, 0.f};
+    float zo[3] = {0.f, 0.f, 0.f}, qo[3] = {0.f, 0.f, 0.f};
+    float4 f0 = make_float4(0.f, 0.f, 0.f, 0.f), f1 = f0;
+    int32_t e1 = 0, kk0 = 0, ent0 = 0;
+    if (row_first < n) {
+        const int64_t o = 3 * row_first;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zi[c] = z[o + c];
+            qi[c] = p_old[o + c];
+        }
+        kk0 = xptr[row_first] + lane;
+        e1 = xptr[row_first + 1];
+        if (kk0 < e1) {
+            ent0 = xlist[kk0];
+            const int64_t oo = 3 * static_cast<int64_t>(xoth[kk0]);
+            f0 = frec[2 * kk0];
+            f1 = frec[2 * kk0 + 1];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                zo[c] = z[oo + c];
+                qo[c] = p_old[oo + c];
+            }
+        }
+    }
+    double red_v[3];
+    red_v[0] = strided_sum<kThreads, 2>(pr_cur, np_cur, 0.0);
+    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
+    red_v[2] = k > 0 ? strided_sum<kThreads, 2>(pr_prev + kPartStride, np_prev, 0.0) : 0.0;
+    block_allsum<3, kThreads>(red_v, red);
     if (status->done) return;
-    double cur[2];
-    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
+    const double rr = red_v[0];
     if (k == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            status->bnorm = sqrt(cur[0]);
-            status->tol = tol_rel * sqrt(cur[0]);
-            status->rnorm = sqrt(cur[0]);
+            status->bnorm = sqrt(rr);
+            status->tol = tol_rel * sqrt(rr);
+            status->rnorm = sqrt(rr);
         }
     } else if (is_true) {
         const double tol = status->tol;
-        if (sqrt(cur[0]) < tol) {
+        if (sqrt(rr) < tol) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 status->done = 1;
                 status->converged = 1;
                 status->iters = k;
-                status->rnorm = sqrt(cur[0]);
+                status->rnorm = sqrt(rr);
             }
             return;
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) status->rnorm = sqrt(cur[0]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) status->rnorm = sqrt(rr);
     }
-    float beta = 0.f;
-    if (k > 0) {
-        double prev[2];
-        sum_partials<2>(pr_prev, np_prev, kPartStride, prev);
-        beta = static_cast<float>(cur[1] / prev[1]);
-    }
-    constexpr int kRowsPerTile = kThreads / G;
-    const int lane = threadIdx.x % G;
+    const float beta = k > 0 ? static_cast<float>(red_v[1] / red_v[2]) : 0.f;
     double pq[1] = {0.0};
     for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
          row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
         const int64_t row = row0 + threadIdx.x / G;
+        const bool first = row == row_first;
         float y[3] = {0.f, 0.f, 0.f};
         float pi[3] = {0.f, 0.f, 0.f};
         if (row < n) {
             const int64_t o = 3 * row;
+            if (first) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) pi[c] = z[o + c] + beta * p_old[o + c];
-            const int32_t e1 = xptr[row + 1];
-            for (int32_t kk = xptr[row] + lane; kk < e1; kk += G) {
-                const int32_t ent = xlist[kk];
+                for (int c = 0; c < 3; ++c) pi[c] = zi[c] + beta * qi[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pi[c] = z[o + c] + beta * p_old[o + c];
+            }
+            const int32_t kk_start = first ? kk0 : xptr[row] + lane;
+            const int32_t kk_end = first ? e1 : xptr[row + 1];
+            for (int32_t kk = kk_start; kk < kk_end; kk += G) {
+                const bool pre = first && kk == kk0;
+                const int32_t ent = pre ? ent0 : xlist[kk];
                 const int64_t e = ent >> 1;
                 const int side = ent & 1;
                 const int type = e < n_obs_e ? 0 : (e < n_obs_e + n_mot_e ? 1 : 2);
-                const int64_t oo = 3 * static_cast<int64_t>(xoth[kk]);
                 float po[3];
+                if (pre) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) po[c] = z[oo + c] + beta * p_old[oo + c];
-                edge_apply(type, side, frec[2 * kk], frec[2 * kk + 1], pi, po, y);
+                    for (int c = 0; c < 3; ++c) po[c] = zo[c] + beta * qo[c];
+                    edge_apply(type, side, f0, f1, pi, po, y);
+                } else {
+                    const int64_t oo = 3 * static_cast<int64_t>(xoth[kk]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) po[c] = z[oo + c] + beta * p_old[oo + c];
+                    edge_apply(type, side, frec[2 * kk], frec[2 * kk + 1], pi, po, y);
+                }
             }
         }
 #pragma unroll
@@ -238,10 +293,12 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
                      static_cast<double>(pi[2]) * y[2];
         }
     }
-    write_partials<1>(pq, pq_out, kPartStride);
+    write_partials<1>(pq, pq_out, kSpmvMaxGrid);
     if (blockIdx.x == 0 && threadIdx.x == 0) status->iters = k;
 }
 
+// B(k): alpha from the p.q partials, x / r / z update, r.r and r.z partials;
+// the per-pose loads are issued before the partials are reduced.
 __global__ void __launch_bounds__(kThreads)
 mixed_update_kernel(const float *__restrict__ p, const float *__restrict__ q,
                     const float *__restrict__ dinv32, double *__restrict__ x,
@@ -250,30 +307,48 @@ mixed_update_kernel(const float *__restrict__ p, const float *__restrict__ q,
                     const double *__restrict__ pr_cur, int32_t np_cur,
                     double *__restrict__ pr_out, PcgStatus *status)
 {
+    __shared__ double red[2][kThreads / kWave];
+    const int32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    double xp[3];
+    float pp[3], qp[3], rp[3], d[9];
+    auto load = [&](int32_t ii) {
+        const int64_t o = 3 * static_cast<int64_t>(ii);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xp[c] = x[o + c];
+            pp[c] = p[o + c];
+            qp[c] = q[o + c];
+            rp[c] = r[o + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) d[c] = dinv32[9 * static_cast<int64_t>(ii) + c];
+    };
+    if (i0 < n) load(i0);
+    double red_v[2];
+    red_v[0] = strided_sum<kThreads, 8>(pq_in, np_pq, 0.0);
+    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
+    block_allsum<2, kThreads>(red_v, red);
     if (status->done) return;
-    double pq[1], cur[2];
-    sum_partials<1>(pq_in, np_pq, kPartStride, pq);
-    if (pq[0] == 0.0) {
+    if (red_v[0] == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
             status->flags |= 1;
         }
         return;
     }
-    sum_partials<2>(pr_cur, np_cur, kPartStride, cur);
-    const double alpha = cur[1] / pq[0];
+    const double alpha = red_v[1] / red_v[0];
     const float af = static_cast<float>(alpha);
     double v[2] = {0.0, 0.0};
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int32_t i = i0; i < n; i += gridDim.x * blockDim.x) {
+        if (i != i0) load(i);
         const int64_t o = 3 * static_cast<int64_t>(i);
         float rr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            x[o + c] += alpha * static_cast<double>(p[o + c]);
-            rr[c] = r[o + c] - af * q[o + c];
+            x[o + c] = xp[c] + alpha * static_cast<double>(pp[c]);
+            rr[c] = rp[c] - af * qp[c];
             r[o + c] = rr[c];
         }
-        const float *d = dinv32 + 9 * static_cast<int64_t>(i);
         float zz[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) zz[c] = d[3 * c] * rr[0] + d[3 * c + 1] * rr[1] + d[3 * c + 2] * rr[2];
@@ -355,10 +430,10 @@ int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, 
                     gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
-    const unsigned int gs = grid_for(n, kLanesMixed), gu = grid_for(n, 1);
+    const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);
     const int32_t every = o.refresh_every > 0 ? o.refresh_every : 64;
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
-    double *PQ = g.part + kPartOffPQ;
+    double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
     PcgStatus *st = g.status;
     const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
     GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
@@ -419,9 +494,9 @@ int time_mixed_spmv(DeviceGraph &g, int reps, double *us)
         return GSLAM_ERR_STATE;
     }
     const int32_t n = g.h.n;
-    const unsigned int gs = grid_for(n, kLanesMixed), gu = grid_for(n, 1);
+    const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
-    double *PQ = g.part + kPartOffPQ;
+    double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
     const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
     hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        g.xi, n, 1, g.dinv32, g.x, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
