@@ -35,6 +35,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
+#include <utility>
 
 namespace gslam {
 
@@ -42,7 +44,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNb = 128;          // panel width, trailing tile edge and GEMM depth
-constexpr int kPivThreads = 256;  // pivot kernel: 16 x 16 threads, 8 x 8 entries each
+constexpr int kPb = 8;            // pivot kernel: pivots per inner block step
+constexpr int kPivThreads = 5 * 64;  // pivot kernel: four MFMA waves and one pivot wave
 constexpr int kXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // ---------------------------------------------------------------- loading
@@ -143,87 +146,210 @@ bgj_store_dense_kernel(const float *__restrict__ a, int64_t n, int64_t np,
 
 // ---------------------------------------------------------------- block step
 
-// P = A_KK^-1 in FP64 by in-place Gauss-Jordan in one workgroup.  Thread t
-// owns the 8 x 8 sub-block (rows 8(t/16).., columns 8(t%16)..) in registers.
-// Per pivot p the owners publish row p and column p to LDS (double-buffered
-// by pivot parity, so one barrier per pivot suffices), the column owners
-// then set their column-p entries to 0 (row p: 1) and publish 1 in place of
-// the pivot in the row buffer, so that every row r != p takes the same form
-//   m[r][c] -= (col[r] / piv) * row[c]
-// (giving -col[r]/piv in column p) and row p is scaled by 1/piv.  Writes
-// A_KK = P and R_K = P (FP32) and PT = P^T (FP64) for the row-panel kernel.
-// A zero pivot sets *singular (the whole inverse is then reported as zero,
+// 1/d to within an ulp: the hardware reciprocal refined by two
+// Newton-Raphson steps r <- r + r (1 - d r).
+__device__ inline double recip_nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+// v of lane `src` in every lane (scalar broadcast of both halves).
+__device__ inline double readlane_f64(double v, int src)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), src);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                          static_cast<unsigned int>(lo));
+}
+
+// In-place Gauss-Jordan of an 8 x 8 block held one entry per lane (lane
+// 8i + j holds entry (i, j)) of a full wave: for each pivot p, row p is
+// scaled by 1/piv, column p becomes -a_ip/piv, the pivot 1/piv, and every
+// other entry a_ij - (a_ip/piv) a_pj.  A zero pivot sets *zero and is taken
+// as 1 (the caller then reports the whole inverse as zero).
+__device__ inline double gj8_wave(double v, int i, int j, bool &zero)
+{
+#pragma unroll
+    for (int p = 0; p < kPb; ++p) {
+        const double rowv = __shfl(v, 8 * p + j, kWave);
+        const double colv = __shfl(v, 8 * i + p, kWave);
+        double piv = readlane_f64(v, 9 * p);
+        if (piv == 0.0) {
+            zero = true;
+            piv = 1.0;
+        }
+        const double ip = recip_nr(piv);
+        if (i == p) v = (j == p) ? ip : v * ip;
+        else if (j == p) v = -(colv * ip);
+        else v = fma(-(colv * ip), rowv, v);
+    }
+    return v;
+}
+
+template <int... I, class F>
+__device__ inline void static_for(std::integer_sequence<int, I...>, F &&f)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+// P = A_KK^-1 (128 x 128) in FP64 by Gauss-Jordan in blocks of 8 pivots,
+// one workgroup of five waves.  Waves 0-3 hold the block in FP64 MFMA
+// accumulator layout (wave w: rows 32w..32w+31, 2 x 8 tiles of 16 x 16;
+// register e of lane l in a tile holds (l/16 + 4e, l%16)).  Step pb
+// (pivots 8pb..8pb+7, "K"; P8 = A_KK^-1 made by wave 4 one step ahead):
+//   1. waves 0-3 publish column panel A[:, K], row panel A[K, :] with I
+//      added to its A_KK part, and the next pivot block to LDS; barrier;
+//   2. waves 0-3: A += C B with FP64 MFMA 16x16x4 (rank 8), where
+//      C[i, :] = -A[i, K] P8 (i not in K), C[K, :] = P8 - I and
+//      B = A[K, :] + [0 I 0], which in one update gives the trailing
+//      entries A_ij - A_iK P8 A_Kj, row K = P8 A_Kj, column K = -A_iK P8 and
+//      block KK = P8; meanwhile wave 4 applies the step to the next pivot
+//      block and inverts it (gj8_wave), so the serial pivot chain runs
+//      beside the MFMAs.
+// LDS buffers alternate with the parity of pb: one barrier per step.  Writes
+// A_KK = P and R_K = P (FP32) and PT = P^T (FP64) for the row-panel kernel;
+// a zero pivot sets *singular (the whole inverse is then reported as zero,
 // MatrixOps.c:60-68).
 __global__ void __launch_bounds__(kPivThreads)
 bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
                  float *__restrict__ rp, int32_t *__restrict__ singular)
 {
-    __shared__ double rowbuf[2][kNb], colbuf[2][kNb], pivbuf[2];
-    const int t = threadIdx.x;
-    const int rb = t >> 4, cb = t & 15;
+    __shared__ double colraw[2][kNb][kPb];  // A[:, K]
+    __shared__ double rowraw[2][kPb][kNb];  // A[K, :] + [0 I 0]
+    __shared__ double pblk[2][kPb][kPb];    // P8 of step pb (parity pb)
+    __shared__ double nextblk[2][kPb][kPb]; // A[K+1, K+1] before step pb
+    __shared__ double rnext[kPb][kPb];      // wave 4: (P8 A[K, K+1])
+    constexpr int kSteps = kNb / kPb;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int l16 = lane & 15, h = lane >> 4;
+    const int pi = lane >> 3, pj = lane & 7;  // wave 4: entry of the 8 x 8 block
     const int64_t k0 = static_cast<int64_t>(kp) * kNb;
-    double m[8][8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float *src = a + (k0 + 8 * rb + i) * np + k0 + 8 * cb;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) m[i][j] = static_cast<double>(src[j]);
-    }
+    f64x4 acc[2][8];
     bool zero = false;
-    for (int pb = 0; pb < kNb / 8; ++pb) {
+    if (wave < 4) {
 #pragma unroll
-        for (int pi = 0; pi < 8; ++pi) {
-            const int buf = pi & 1;
-            if (rb == pb) {
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    rowbuf[buf][8 * cb + j] = (j == pi && cb == pb) ? 1.0 : m[pi][j];
+            for (int y = 0; y < 8; ++y)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t row = 32 * wave + 16 * x + h + 4 * e;
+                    acc[x][y][e] = static_cast<double>(a[(k0 + row) * np + k0 + 16 * y + l16]);
+                }
+    } else {
+        const double v = static_cast<double>(a[(k0 + pi) * np + k0 + pj]);
+        pblk[0][pi][pj] = gj8_wave(v, pi, pj, zero);
+    }
+    __syncthreads();
+    static_for(std::make_integer_sequence<int, kSteps>{}, [&](auto pbc) {
+        constexpr int pb = decltype(pbc)::value;
+        constexpr int buf = pb & 1;
+        constexpr int yk = pb >> 1, half = pb & 1;       // column tile / lane half of K
+        constexpr int xk = (pb >> 1) & 1, wk = pb >> 2;  // row tile / wave of K
+        const bool lane_in_k = (l16 >> 3) == half;
+        // 1. publish
+        if (wave < 4) {
+            if (lane_in_k) {
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        colraw[buf][32 * wave + 16 * x + h + 4 * e][l16 - 8 * half] = acc[x][yk][e];
             }
-            if (cb == pb) {
+            if (wave == wk) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) colbuf[buf][8 * rb + i] = m[i][pi];
-                if (rb == pb) pivbuf[buf] = m[pi][pi];
+                for (int y = 0; y < 8; ++y)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) m[i][pi] = (i == pi && rb == pb) ? 1.0 : 0.0;
+                    for (int ee = 0; ee < 2; ++ee) {
+                        const int e = 2 * half + ee, r = h + 4 * ee;  // row of K
+                        double v = acc[xk][y][e];
+                        if (y == yk && lane_in_k && r == l16 - 8 * half) v += 1.0;
+                        rowraw[buf][r][16 * y + l16] = v;
+                    }
             }
-            __syncthreads();
-            double piv = pivbuf[buf];
-            if (piv == 0.0) {
-                zero = true;
-                piv = 1.0;
-            }
-            const double ip = 1.0 / piv;
-            double rv[8], cv[8];
+            if constexpr (pb + 1 < kSteps) {
+                constexpr int q = pb + 1;
+                constexpr int yq = q >> 1, hq = q & 1, xq = (q >> 1) & 1, wq = q >> 2;
+                if (wave == wq && (l16 >> 3) == hq) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) rv[j] = rowbuf[buf][8 * cb + j];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) cv[i] = colbuf[buf][8 * rb + i];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (i == pi && rb == pb) {
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) m[i][j] *= ip;
-                } else {
-                    const double f = cv[i] * ip;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) m[i][j] = fma(-f, rv[j], m[i][j]);
+                    for (int ee = 0; ee < 2; ++ee)
+                        nextblk[buf][h + 4 * ee][l16 - 8 * hq] = acc[xq][yq][2 * hq + ee];
                 }
             }
         }
-    }
-    if (zero && t == 0) *singular = 1;
+        __syncthreads();
+        if (wave < 4) {
+            // 2. C in the MFMA A layout (lane l: row 16x + l%16, k = 4c + l/16)
+            double pcol[2][kPb];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = 8 * rb + i;
-        float *dst = a + (k0 + r) * np + k0 + 8 * cb;
-        float *rdst = rp + static_cast<int64_t>(r) * np + k0 + 8 * cb;
+            for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            dst[j] = static_cast<float>(m[i][j]);
-            rdst[j] = static_cast<float>(m[i][j]);
-            pt[(8 * cb + j) * kNb + r] = m[i][j];
+                for (int m = 0; m < kPb; ++m) pcol[c][m] = pblk[buf][m][4 * c + h];
+            double av[2][2], bv[8][2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const int r = 32 * wave + 16 * x + l16;
+                double cr[kPb];
+#pragma unroll
+                for (int m = 0; m < kPb; ++m) cr[m] = colraw[buf][r][m];
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int m = 0; m < kPb; ++m) s = fma(cr[m], pcol[c][m], s);
+                    av[x][c] = -s;
+                    if (wave == wk && x == xk && lane_in_k)  // row r of K: (P8 - I)
+                        av[x][c] = pblk[buf][r & 7][4 * c + h] - ((r & 7) == 4 * c + h ? 1.0 : 0.0);
+                }
+            }
+#pragma unroll
+            for (int y = 0; y < 8; ++y)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) bv[y][c] = rowraw[buf][4 * c + h][16 * y + l16];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 8; ++y)
+                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][c], bv[y][c], acc[x][y],
+                                                                       0, 0, 0);
+        } else if constexpr (pb + 1 < kSteps) {
+            // next pivot block: A[K+1, K+1] - A[K+1, K] P8 A[K, K+1], inverted
+            constexpr int n0 = kPb * (pb + 1);
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < kPb; ++k) r = fma(pblk[buf][pi][k], rowraw[buf][k][n0 + pj], r);
+            rnext[pi][pj] = r;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double v = nextblk[buf][pi][pj];
+#pragma unroll
+            for (int m = 0; m < kPb; ++m) v = fma(-colraw[buf][n0 + pi][m], rnext[m][pj], v);
+            pblk[buf ^ 1][pi][pj] = gj8_wave(v, pi, pj, zero);
         }
+    });
+    if (wave == 4) {
+        if (zero && lane == 0) *singular = 1;
+        return;
     }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 8; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = 32 * wave + 16 * x + h + 4 * e;
+                const int64_t col = 16 * y + l16;
+                const double v = acc[x][y][e];
+                a[(k0 + row) * np + k0 + col] = static_cast<float>(v);
+                rp[row * np + k0 + col] = static_cast<float>(v);
+                pt[col * kNb + row] = v;
+            }
 }
 
 // Row panel and column panel of block step kp.
@@ -295,67 +421,149 @@ bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__
 
 // Trailing update of block step kp: every 128 x 128 tile (ti, tj), ti != kp,
 //   A_tile = (tj == kp ? 0 : A_tile) + CT[:, ti]^T R[:, tj]
-// with MFMA 32x32x2 FP32 (FP32 accumulate).  Four waves per workgroup, a 64
-// x 64 quadrant each (2 x 2 MFMA blocks, 64 accumulators per lane); the
-// operands come straight from L2 in MFMA lane layout (CT and R are k-major,
-// so lanes 0-31 and 32-63 each read 128 contiguous bytes) and the old tile
-// is loaded into the accumulators before the k loop.  Workgroups are dealt
-// to the 8 XCDs round-robin; tile t = xcd * per_xcd + slot keeps the tiles of
-// one block row (one CT slab) on one XCD's L2.
-__global__ void __launch_bounds__(kThreads)
+// with MFMA 32x32x2 FP32 (FP32 accumulate).  Persistent workgroups of eight
+// waves, one per CU, each walking its tiles:
+//  * waves 4-7 stream the operands -- the 128 x kTrKc chunks of the CT slab
+//    (block row ti) and of the R slab (block column tj), k-major -- into
+//    double-buffered LDS, one chunk ahead (global loads held in registers
+//    across a chunk, then written to the buffer the MFMA waves left);
+//  * waves 0-3 (a 64 x 64 quadrant each, 2 x 2 MFMA blocks) issue the loads
+//    of the old tile as the tile starts and accumulate the first chunk's
+//    products from zero while they land (operands from LDS only); the old
+//    tile is then added and the other chunks accumulate onto it, so the
+//    HBM traffic of a tile overlaps its MFMAs.
+// One barrier per chunk.  Tiles are grouped per XCD (workgroup w runs on XCD
+// w % 8; XCD x takes tiles [x per_xcd, (x+1) per_xcd)), which keeps the tiles
+// of one block row, and its CT slab, on one XCD's L2.
+constexpr int kTrKc = 32;                 // k rows per operand chunk
+constexpr int kTrChunks = kNb / kTrKc;    // chunks per tile
+constexpr int kTrThreads = 8 * 64;        // 4 MFMA waves + 4 loader waves
+constexpr int kTrLd = kNb + 32;           // LDS row stride (floats): the two
+                                          // half-waves read different banks
+constexpr int kTrWgPerXcd = 32;           // one workgroup per CU (256 CUs)
+
+__device__ inline void trailing_tile(int32_t t, int32_t nt, int32_t kp, int32_t &ti, int32_t &tj)
+{
+    ti = t / nt;
+    tj = t - ti * nt;
+    if (ti >= kp) ++ti;
+}
+
+__global__ void __launch_bounds__(kTrThreads)
 bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
                     const float *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
                     int32_t per_xcd)
 {
-    const int32_t w = blockIdx.x;
-    const int32_t t = (w % kXcd) * per_xcd + w / kXcd;
-    if (t >= (nt - 1) * nt) return;
-    int32_t ti = t / nt;
-    const int32_t tj = t - ti * nt;
-    if (ti >= kp) ++ti;
+    __shared__ __attribute__((aligned(16))) float ctl[2][kTrKc][kTrLd];
+    __shared__ __attribute__((aligned(16))) float rl[2][kTrKc][kTrLd];
+    const int32_t n_tiles = (nt - 1) * nt;
+    const int32_t base = static_cast<int32_t>(blockIdx.x % kXcd) * per_xcd;
+    const int32_t end = min(base + per_xcd, n_tiles);
+    const int32_t first = base + static_cast<int32_t>(blockIdx.x / kXcd);
+    const int32_t stride = static_cast<int32_t>(gridDim.x / kXcd);
+    const int32_t n_my = first < end ? (end - first + stride - 1) / stride : 0;
+    const int32_t n_chunks = n_my * kTrChunks;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (wave >= 4) {
+        // loader: chunk g = (tile g / kTrChunks, rows kTrKc (g % kTrChunks)..)
+        const int lt = threadIdx.x - 4 * kWave;
+        constexpr int kPer = kTrKc * kNb / 4 / (4 * kWave);  // float4 per thread per slab
+        static_assert(kPer == 4, "loader holds four float4 per slab");
+        float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+        const int row0 = lt / (kNb / 4), col0 = 4 * (lt % (kNb / 4));  // q-th: row0 + 8q
+        constexpr int kRowStep = 4 * kWave / (kNb / 4);
+        const float *pct = nullptr, *prp = nullptr;
+#define GSLAM_TR_LOAD(g)                                                                         \
+    {                                                                                            \
+        int32_t ti_, tj_;                                                                        \
+        trailing_tile(first + ((g) / kTrChunks) * stride, nt, kp, ti_, tj_);                     \
+        const int64_t k0_ = static_cast<int64_t>((g) % kTrChunks) * kTrKc + row0;                 \
+        pct = ct + k0_ * np + static_cast<int64_t>(ti_) * kNb + col0;                            \
+        prp = rp + k0_ * np + static_cast<int64_t>(tj_) * kNb + col0;                            \
+        ra0 = *reinterpret_cast<const float4 *>(pct);                                            \
+        ra1 = *reinterpret_cast<const float4 *>(pct + kRowStep * np);                            \
+        ra2 = *reinterpret_cast<const float4 *>(pct + 2 * kRowStep * np);                        \
+        ra3 = *reinterpret_cast<const float4 *>(pct + 3 * kRowStep * np);                        \
+        rb0 = *reinterpret_cast<const float4 *>(prp);                                            \
+        rb1 = *reinterpret_cast<const float4 *>(prp + kRowStep * np);                            \
+        rb2 = *reinterpret_cast<const float4 *>(prp + 2 * kRowStep * np);                        \
+        rb3 = *reinterpret_cast<const float4 *>(prp + 3 * kRowStep * np);                        \
+    }
+#define GSLAM_TR_STORE(b)                                                                        \
+    {                                                                                            \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0][col0]) = ra0;                                \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + kRowStep][col0]) = ra1;                     \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 2 * kRowStep][col0]) = ra2;                 \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 3 * kRowStep][col0]) = ra3;                 \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0][col0]) = rb0;                                 \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + kRowStep][col0]) = rb1;                      \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 2 * kRowStep][col0]) = rb2;                  \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 3 * kRowStep][col0]) = rb3;                  \
+    }
+        if (n_chunks > 0) {
+            GSLAM_TR_LOAD(0)
+            GSLAM_TR_STORE(0)
+        }
+        if (n_chunks > 1) GSLAM_TR_LOAD(1)
+        __syncthreads();
+        for (int32_t g = 0; g < n_chunks; ++g) {
+            if (g + 1 < n_chunks) GSLAM_TR_STORE((g + 1) & 1)
+            if (g + 2 < n_chunks) GSLAM_TR_LOAD(g + 2)
+            __syncthreads();
+        }
+#undef GSLAM_TR_LOAD
+#undef GSLAM_TR_STORE
+        return;
+    }
     const int l32 = lane & 31, h = lane >> 5;
-    const int64_t i0 = static_cast<int64_t>(ti) * kNb + 64 * (wave >> 1);
-    const int64_t j0 = static_cast<int64_t>(tj) * kNb + 64 * (wave & 1);
-    f32x16 acc[2][2];
-    float *cbase = a + (i0 + 4 * h) * np + j0 + l32;
-    if (tj != kp) {
+    const int qi = 64 * (wave >> 1), qj = 64 * (wave & 1);
+    __syncthreads();
+    for (int32_t it = 0; it < n_my; ++it) {
+        int32_t ti, tj;
+        trailing_tile(first + it * stride, nt, kp, ti, tj);
+        float *cbase = a + (static_cast<int64_t>(ti) * kNb + qi + 4 * h) * np +
+                       static_cast<int64_t>(tj) * kNb + qj + l32;
+        f32x16 cold[2][2], acc[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    cold[x][y][e] = tj != kp ? cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y] : 0.f;
+                    acc[x][y][e] = 0.f;
+                }
+        for (int c = 0; c < kTrChunks; ++c) {
+            const int b = (it * kTrChunks + c) & 1;
+            const float *pa = &ctl[b][h][qi + l32];
+            const float *pb = &rl[b][h][qj + l32];
+#pragma unroll
+            for (int s2 = 0; s2 < kTrKc / 2; ++s2) {
+                const float a0 = pa[2 * s2 * kTrLd], a1 = pa[2 * s2 * kTrLd + 32];
+                const float b0 = pb[2 * s2 * kTrLd], b1 = pb[2 * s2 * kTrLd + 32];
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+            }
+            __syncthreads();
+            if (c == 0) {
+                // the old tile joins the sum after the first chunk, so the
+                // remaining products accumulate onto it as in an in-place update
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) acc[x][y] += cold[x][y];
+            }
+        }
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
-                    acc[x][y][e] = cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y];
-    } else {
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[x][y][e] = 0.f;
+                    cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y] = acc[x][y][e];
     }
-    const float *pa = ct + static_cast<int64_t>(h) * np + i0 + l32;
-    const float *pb = rp + static_cast<int64_t>(h) * np + j0 + l32;
-    // (an explicit 8-step-deep operand prefetch measured 10% slower than
-    // this loop, whose loads the compiler issues one step ahead)
-#pragma unroll 8
-    for (int s = 0; s < kNb / 2; ++s) {
-        const int64_t o = 2 * static_cast<int64_t>(s) * np;
-        const float a0 = pa[o], a1 = pa[o + 32];
-        const float b0 = pb[o], b1 = pb[o + 32];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
-    }
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int e = 0; e < 16; ++e)
-                cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y] = acc[x][y][e];
 }
 
 // ---------------------------------------------------------------- refinement
@@ -578,6 +786,17 @@ void DenseWork::release()
     n = np = np_cap = bytes = 0;
 }
 
+// Trailing-update launch shape: tiles per XCD and persistent workgroups
+// (at most one per CU, no more than there are tiles per XCD).
+static int32_t trailing_per_xcd(int32_t nt)
+{
+    return ((nt - 1) * nt + kXcd - 1) / kXcd;
+}
+static unsigned int trailing_grid(int32_t nt)
+{
+    return static_cast<unsigned int>(kXcd * std::min(kTrWgPerXcd, trailing_per_xcd(nt)));
+}
+
 // One block step's three launches.
 static void bgj_step(hipStream_t s, DenseWork &w, int32_t kp)
 {
@@ -588,10 +807,8 @@ static void bgj_step(hipStream_t s, DenseWork &w, int32_t kp)
     const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
     hipLaunchKernelGGL(bgj_panel_kernel, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp, w.pt,
                        w.rp, w.ct);
-    const int32_t tiles = (nt - 1) * nt;
-    const int32_t per = (tiles + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL(bgj_trailing_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
-                       w.rp, w.np, nt, kp, per);
+    hipLaunchKernelGGL(bgj_trailing_kernel, dim3(trailing_grid(nt)), dim3(kTrThreads), 0, s, w.a,
+                       w.ct, w.rp, w.np, nt, kp, trailing_per_xcd(nt));
 }
 
 // In-place FP32 blocked inverse of w.a (np x np, padded).  *singular set on
@@ -613,11 +830,9 @@ int bgj_time_trailing(hipStream_t s, DenseWork &w, int reps, hipEvent_t e0, hipE
         set_error("trailing update timing needs 3N > 128");
         return GSLAM_ERR_STATE;
     }
-    const int32_t tiles = (nt - 1) * nt;
-    const int32_t per = (tiles + kXcd - 1) / kXcd;
     auto launch = [&]() {
-        hipLaunchKernelGGL(bgj_trailing_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
-                           w.rp, w.np, nt, 0, per);
+        hipLaunchKernelGGL(bgj_trailing_kernel, dim3(trailing_grid(nt)), dim3(kTrThreads), 0, s,
+                           w.a, w.ct, w.rp, w.np, nt, 0, trailing_per_xcd(nt));
     };
     for (int i = 0; i < 3; ++i) launch();
     GSLAM_HIP(hipGetLastError());
