@@ -425,16 +425,16 @@ bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__
 // waves, one per CU, each walking its tiles:
 //  * waves 4-7 stream the operands -- the 128 x kTrKc chunks of the CT slab
 //    (block row ti) and of the R slab (block column tj), k-major -- into
-//    double-buffered LDS, one chunk ahead (global loads held in registers
-//    across a chunk, then written to the buffer the MFMA waves left);
+//    double-buffered LDS (global loads issued two chunks ahead into one of
+//    two register sets, written to the buffer the MFMA waves just left);
 //  * waves 0-3 (a 64 x 64 quadrant each, 2 x 2 MFMA blocks) issue the loads
 //    of the old tile as the tile starts and accumulate the first chunk's
 //    products from zero while they land (operands from LDS only); the old
 //    tile is then added and the other chunks accumulate onto it, so the
 //    HBM traffic of a tile overlaps its MFMAs.
 // One barrier per chunk.  Tiles are grouped per XCD (workgroup w runs on XCD
-// w % 8; XCD x takes tiles [x per_xcd, (x+1) per_xcd)), which keeps the tiles
-// of one block row, and its CT slab, on one XCD's L2.
+// w % 8; XCD x takes tiles [x per_xcd, (x+1) per_xcd) of the strip order of
+// trailing_tile), so the CT and R slabs in use stay in one XCD's L2.
 constexpr int kTrKc = 32;                 // k rows per operand chunk
 constexpr int kTrChunks = kNb / kTrKc;    // chunks per tile
 constexpr int kTrThreads = 8 * 64;        // 4 MFMA waves + 4 loader waves
@@ -442,10 +442,16 @@ constexpr int kTrLd = kNb + 32;           // LDS row stride (floats): the two
                                           // half-waves read different banks
 constexpr int kTrWgPerXcd = 32;           // one workgroup per CU (256 CUs)
 
+// Tile t of block step kp -> (ti, tj): the nt - 1 block rows other than kp
+// are taken in strips of four, each strip column by column, so that the
+// tiles an XCD works on at one time share few CT and R slabs.
 __device__ inline void trailing_tile(int32_t t, int32_t nt, int32_t kp, int32_t &ti, int32_t &tj)
 {
-    ti = t / nt;
-    tj = t - ti * nt;
+    const int32_t strip = t / (4 * nt);
+    const int32_t hgt = min(4, nt - 1 - 4 * strip);
+    const int32_t off = t - strip * 4 * nt;
+    tj = off / hgt;
+    ti = 4 * strip + (off - tj * hgt);
     if (ti >= kp) ++ti;
 }
 
@@ -469,46 +475,56 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
         const int lt = threadIdx.x - 4 * kWave;
         constexpr int kPer = kTrKc * kNb / 4 / (4 * kWave);  // float4 per thread per slab
         static_assert(kPer == 4, "loader holds four float4 per slab");
-        float4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+        // two register sets: the loads of chunk g + 2 are issued while chunk g
+        // is multiplied, and land in LDS as the MFMA waves move to chunk g + 1
+        float4 ra0_0, ra0_1, ra0_2, ra0_3, rb0_0, rb0_1, rb0_2, rb0_3;
+        float4 ra1_0, ra1_1, ra1_2, ra1_3, rb1_0, rb1_1, rb1_2, rb1_3;
         const int row0 = lt / (kNb / 4), col0 = 4 * (lt % (kNb / 4));  // q-th: row0 + 8q
         constexpr int kRowStep = 4 * kWave / (kNb / 4);
-        const float *pct = nullptr, *prp = nullptr;
-#define GSLAM_TR_LOAD(g)                                                                         \
+#define GSLAM_TR_LOAD(g, S)                                                                      \
     {                                                                                            \
         int32_t ti_, tj_;                                                                        \
         trailing_tile(first + ((g) / kTrChunks) * stride, nt, kp, ti_, tj_);                     \
         const int64_t k0_ = static_cast<int64_t>((g) % kTrChunks) * kTrKc + row0;                 \
-        pct = ct + k0_ * np + static_cast<int64_t>(ti_) * kNb + col0;                            \
-        prp = rp + k0_ * np + static_cast<int64_t>(tj_) * kNb + col0;                            \
-        ra0 = *reinterpret_cast<const float4 *>(pct);                                            \
-        ra1 = *reinterpret_cast<const float4 *>(pct + kRowStep * np);                            \
-        ra2 = *reinterpret_cast<const float4 *>(pct + 2 * kRowStep * np);                        \
-        ra3 = *reinterpret_cast<const float4 *>(pct + 3 * kRowStep * np);                        \
-        rb0 = *reinterpret_cast<const float4 *>(prp);                                            \
-        rb1 = *reinterpret_cast<const float4 *>(prp + kRowStep * np);                            \
-        rb2 = *reinterpret_cast<const float4 *>(prp + 2 * kRowStep * np);                        \
-        rb3 = *reinterpret_cast<const float4 *>(prp + 3 * kRowStep * np);                        \
+        const float *pct = ct + k0_ * np + static_cast<int64_t>(ti_) * kNb + col0;               \
+        const float *prp = rp + k0_ * np + static_cast<int64_t>(tj_) * kNb + col0;               \
+        ra##S##_0 = *reinterpret_cast<const float4 *>(pct);                                       \
+        ra##S##_1 = *reinterpret_cast<const float4 *>(pct + kRowStep * np);                       \
+        ra##S##_2 = *reinterpret_cast<const float4 *>(pct + 2 * kRowStep * np);                   \
+        ra##S##_3 = *reinterpret_cast<const float4 *>(pct + 3 * kRowStep * np);                   \
+        rb##S##_0 = *reinterpret_cast<const float4 *>(prp);                                       \
+        rb##S##_1 = *reinterpret_cast<const float4 *>(prp + kRowStep * np);                       \
+        rb##S##_2 = *reinterpret_cast<const float4 *>(prp + 2 * kRowStep * np);                   \
+        rb##S##_3 = *reinterpret_cast<const float4 *>(prp + 3 * kRowStep * np);                   \
     }
-#define GSLAM_TR_STORE(b)                                                                        \
+#define GSLAM_TR_STORE(b, S)                                                                     \
     {                                                                                            \
-        *reinterpret_cast<float4 *>(&ctl[(b)][row0][col0]) = ra0;                                \
-        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + kRowStep][col0]) = ra1;                     \
-        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 2 * kRowStep][col0]) = ra2;                 \
-        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 3 * kRowStep][col0]) = ra3;                 \
-        *reinterpret_cast<float4 *>(&rl[(b)][row0][col0]) = rb0;                                 \
-        *reinterpret_cast<float4 *>(&rl[(b)][row0 + kRowStep][col0]) = rb1;                      \
-        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 2 * kRowStep][col0]) = rb2;                  \
-        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 3 * kRowStep][col0]) = rb3;                  \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0][col0]) = ra##S##_0;                           \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + kRowStep][col0]) = ra##S##_1;                \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 2 * kRowStep][col0]) = ra##S##_2;            \
+        *reinterpret_cast<float4 *>(&ctl[(b)][row0 + 3 * kRowStep][col0]) = ra##S##_3;            \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0][col0]) = rb##S##_0;                            \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + kRowStep][col0]) = rb##S##_1;                 \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 2 * kRowStep][col0]) = rb##S##_2;             \
+        *reinterpret_cast<float4 *>(&rl[(b)][row0 + 3 * kRowStep][col0]) = rb##S##_3;             \
     }
+        // chunk c sits in register set c & 1 and LDS buffer c & 1
         if (n_chunks > 0) {
-            GSLAM_TR_LOAD(0)
-            GSLAM_TR_STORE(0)
+            GSLAM_TR_LOAD(0, 0)
+            GSLAM_TR_STORE(0, 0)
         }
-        if (n_chunks > 1) GSLAM_TR_LOAD(1)
+        if (n_chunks > 1) GSLAM_TR_LOAD(1, 1)
+        if (n_chunks > 2) GSLAM_TR_LOAD(2, 0)
         __syncthreads();
-        for (int32_t g = 0; g < n_chunks; ++g) {
-            if (g + 1 < n_chunks) GSLAM_TR_STORE((g + 1) & 1)
-            if (g + 2 < n_chunks) GSLAM_TR_LOAD(g + 2)
+        for (int32_t g = 0; g < n_chunks; g += 2) {
+            // g even: store chunk g + 1 (set 1), load chunk g + 3 (set 1)
+            if (g + 1 < n_chunks) GSLAM_TR_STORE(1, 1)
+            if (g + 3 < n_chunks) GSLAM_TR_LOAD(g + 3, 1)
+            __syncthreads();
+            if (g + 1 >= n_chunks) break;
+            // g + 1 odd: store chunk g + 2 (set 0), load chunk g + 4 (set 0)
+            if (g + 2 < n_chunks) GSLAM_TR_STORE(0, 0)
+            if (g + 4 < n_chunks) GSLAM_TR_LOAD(g + 4, 0)
             __syncthreads();
         }
 #undef GSLAM_TR_LOAD
