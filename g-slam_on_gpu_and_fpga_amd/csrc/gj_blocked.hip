@@ -213,15 +213,24 @@ __device__ inline void static_for(std::integer_sequence<int, I...>, F &&f)
 // A_KK = P and R_K = P (FP32) and PT = P^T (FP64) for the row-panel kernel;
 // a zero pivot sets *singular (the whole inverse is then reported as zero,
 // MatrixOps.c:60-68).
-__global__ void __launch_bounds__(kPivThreads)
-bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
-                 float *__restrict__ rp, int32_t *__restrict__ singular)
+struct PivLds {
+    double colraw[2][kNb][kPb];  // A[:, K]
+    double rowraw[2][kPb][kNb];  // A[K, :] + [0 I 0]
+    double pblk[2][kPb][kPb];    // P8 of step pb (parity pb)
+    double nextblk[2][kPb][kPb]; // A[K+1, K+1] before step pb
+    double rnext[kPb][kPb];      // wave 4: (P8 A[K, K+1])
+};
+
+// Waves beyond the fifth take part in the barriers only.
+__device__ __attribute__((always_inline)) inline void pivot_block(float *__restrict__ a, int64_t np, int32_t kp,
+                                   double *__restrict__ pt, float *__restrict__ rp,
+                                   int32_t *__restrict__ singular, PivLds &L)
 {
-    __shared__ double colraw[2][kNb][kPb];  // A[:, K]
-    __shared__ double rowraw[2][kPb][kNb];  // A[K, :] + [0 I 0]
-    __shared__ double pblk[2][kPb][kPb];    // P8 of step pb (parity pb)
-    __shared__ double nextblk[2][kPb][kPb]; // A[K+1, K+1] before step pb
-    __shared__ double rnext[kPb][kPb];      // wave 4: (P8 A[K, K+1])
+    auto &colraw = L.colraw;
+    auto &rowraw = L.rowraw;
+    auto &pblk = L.pblk;
+    auto &nextblk = L.nextblk;
+    auto &rnext = L.rnext;
     constexpr int kSteps = kNb / kPb;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int l16 = lane & 15, h = lane >> 4;
@@ -239,7 +248,7 @@ bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restri
                     const int64_t row = 32 * wave + 16 * x + h + 4 * e;
                     acc[x][y][e] = static_cast<double>(a[(k0 + row) * np + k0 + 16 * y + l16]);
                 }
-    } else {
+    } else if (wave == 4) {
         const double v = static_cast<double>(a[(k0 + pi) * np + k0 + pj]);
         pblk[0][pi][pj] = gj8_wave(v, pi, pj, zero);
     }
@@ -317,24 +326,26 @@ bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restri
                     for (int y = 0; y < 8; ++y)
                         acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][c], bv[y][c], acc[x][y],
                                                                        0, 0, 0);
-        } else if constexpr (pb + 1 < kSteps) {
-            // next pivot block: A[K+1, K+1] - A[K+1, K] P8 A[K, K+1], inverted
-            constexpr int n0 = kPb * (pb + 1);
-            double r = 0.0;
+        } else if (wave == 4) {
+            if constexpr (pb + 1 < kSteps) {
+                // next pivot block: A[K+1, K+1] - A[K+1, K] P8 A[K, K+1], inverted
+                constexpr int n0 = kPb * (pb + 1);
+                double r = 0.0;
 #pragma unroll
-            for (int k = 0; k < kPb; ++k) r = fma(pblk[buf][pi][k], rowraw[buf][k][n0 + pj], r);
-            rnext[pi][pj] = r;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            double v = nextblk[buf][pi][pj];
+                for (int k = 0; k < kPb; ++k) r = fma(pblk[buf][pi][k], rowraw[buf][k][n0 + pj], r);
+                rnext[pi][pj] = r;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                double v = nextblk[buf][pi][pj];
 #pragma unroll
-            for (int m = 0; m < kPb; ++m) v = fma(-colraw[buf][n0 + pi][m], rnext[m][pj], v);
-            pblk[buf ^ 1][pi][pj] = gj8_wave(v, pi, pj, zero);
+                for (int m = 0; m < kPb; ++m) v = fma(-colraw[buf][n0 + pi][m], rnext[m][pj], v);
+                pblk[buf ^ 1][pi][pj] = gj8_wave(v, pi, pj, zero);
+            }
         }
     });
-    if (wave == 4) {
-        if (zero && lane == 0) *singular = 1;
+    if (wave >= 4) {
+        if (wave == 4 && zero && lane == 0) *singular = 1;
         return;
     }
 #pragma unroll
@@ -350,6 +361,14 @@ bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restri
                 rp[row * np + k0 + col] = static_cast<float>(v);
                 pt[col * kNb + row] = v;
             }
+}
+
+__global__ void __launch_bounds__(kPivThreads)
+bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
+                 float *__restrict__ rp, int32_t *__restrict__ singular)
+{
+    __shared__ PivLds lds;
+    pivot_block(a, np, kp, pt, rp, singular, lds);
 }
 
 // Row panel and column panel of block step kp.
@@ -432,15 +451,30 @@ bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__
 //    products from zero while they land (operands from LDS only); the old
 //    tile is then added and the other chunks accumulate onto it, so the
 //    HBM traffic of a tile overlaps its MFMAs.
-// One barrier per chunk.  Tiles are grouped per XCD (workgroup w runs on XCD
-// w % 8; XCD x takes tiles [x per_xcd, (x+1) per_xcd) of the strip order of
-// trailing_tile), so the CT and R slabs in use stay in one XCD's L2.
+// One barrier per chunk.  Tiles are grouped per XCD (workgroup b runs on XCD
+// b % 8; the workgroups of one XCD share one contiguous range of the strip
+// order of trailing_tile), so the CT and R slabs in use stay in one XCD's L2.
+//
+// With `fuse`, workgroup 0 instead updates tile (kp+1, kp+1) itself (it is
+// taken out of the other workgroups' tile order) and then inverts that block
+// as the pivot of step kp+1 (pivot_block, writing R_{kp+1} = P into rp_next):
+// the serial pivot inversion runs beside the trailing tiles.
 constexpr int kTrKc = 32;                 // k rows per operand chunk
 constexpr int kTrChunks = kNb / kTrKc;    // chunks per tile
 constexpr int kTrThreads = 8 * 64;        // 4 MFMA waves + 4 loader waves
 constexpr int kTrLd = kNb + 32;           // LDS row stride (floats): the two
                                           // half-waves read different banks
 constexpr int kTrWgPerXcd = 32;           // one workgroup per CU (256 CUs)
+
+struct TrLds {
+    float ctl[2][kTrKc][kTrLd];
+    float rl[2][kTrKc][kTrLd];
+};
+
+union GjLds {
+    TrLds t;
+    PivLds p;
+};
 
 // Tile t of block step kp -> (ti, tj): the nt - 1 block rows other than kp
 // are taken in strips of four, each strip column by column, so that the
@@ -455,19 +489,38 @@ __device__ inline void trailing_tile(int32_t t, int32_t nt, int32_t kp, int32_t 
     if (ti >= kp) ++ti;
 }
 
-__global__ void __launch_bounds__(kTrThreads)
-bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
-                    const float *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
-                    int32_t per_xcd)
+// Index in the tile order with tile t_skip taken out.
+__device__ inline int32_t skip_tile(int32_t t, int32_t t_skip) { return t >= t_skip ? t + 1 : t; }
+
+// Tiles of this workgroup: first, first + stride, ... (n_my of them).  XCD x
+// has wpx workgroups (wpx - fuse for XCD 0, whose workgroup 0 is the pivot
+// one) and the range [n cum(x) / total, n cum(x+1) / total) of the tiles,
+// cum(x) = workgroups on the XCDs before x.
+struct TrPlan {
+    int32_t first, stride, n_my;
+};
+
+__device__ inline TrPlan trailing_plan(int32_t n_tiles, int32_t wpx, int32_t fuse)
 {
-    __shared__ __attribute__((aligned(16))) float ctl[2][kTrKc][kTrLd];
-    __shared__ __attribute__((aligned(16))) float rl[2][kTrKc][kTrLd];
-    const int32_t n_tiles = (nt - 1) * nt;
-    const int32_t base = static_cast<int32_t>(blockIdx.x % kXcd) * per_xcd;
-    const int32_t end = min(base + per_xcd, n_tiles);
-    const int32_t first = base + static_cast<int32_t>(blockIdx.x / kXcd);
-    const int32_t stride = static_cast<int32_t>(gridDim.x / kXcd);
-    const int32_t n_my = first < end ? (end - first + stride - 1) / stride : 0;
+    const int32_t b = static_cast<int32_t>(blockIdx.x), x = b % kXcd;
+    const int32_t total = kXcd * wpx - fuse;
+    const int32_t cum0 = x == 0 ? 0 : x * wpx - fuse, cum1 = (x + 1) * wpx - fuse;
+    const int32_t lo = static_cast<int32_t>(static_cast<int64_t>(n_tiles) * cum0 / total);
+    const int32_t hi = static_cast<int32_t>(static_cast<int64_t>(n_tiles) * cum1 / total);
+    TrPlan pl;
+    pl.stride = (x == 0) ? wpx - fuse : wpx;
+    pl.first = lo + b / kXcd - ((x == 0) ? fuse : 0);
+    pl.n_my = pl.first < hi ? (hi - pl.first + pl.stride - 1) / pl.stride : 0;
+    return pl;
+}
+
+__device__ __attribute__((always_inline)) inline void trailing_tiles(float *__restrict__ a, const float *__restrict__ ct,
+                                      const float *__restrict__ rp, int64_t np, int32_t nt,
+                                      int32_t kp, int32_t t_skip, TrPlan pl, TrLds &L)
+{
+    auto &ctl = L.ctl;
+    auto &rl = L.rl;
+    const int32_t first = pl.first, stride = pl.stride, n_my = pl.n_my;
     const int32_t n_chunks = n_my * kTrChunks;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (wave >= 4) {
@@ -484,7 +537,7 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
 #define GSLAM_TR_LOAD(g, S)                                                                      \
     {                                                                                            \
         int32_t ti_, tj_;                                                                        \
-        trailing_tile(first + ((g) / kTrChunks) * stride, nt, kp, ti_, tj_);                     \
+        trailing_tile(skip_tile(first + ((g) / kTrChunks) * stride, t_skip), nt, kp, ti_, tj_);                     \
         const int64_t k0_ = static_cast<int64_t>((g) % kTrChunks) * kTrKc + row0;                 \
         const float *pct = ct + k0_ * np + static_cast<int64_t>(ti_) * kNb + col0;               \
         const float *prp = rp + k0_ * np + static_cast<int64_t>(tj_) * kNb + col0;               \
@@ -536,7 +589,7 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
     __syncthreads();
     for (int32_t it = 0; it < n_my; ++it) {
         int32_t ti, tj;
-        trailing_tile(first + it * stride, nt, kp, ti, tj);
+        trailing_tile(skip_tile(first + it * stride, t_skip), nt, kp, ti, tj);
         float *cbase = a + (static_cast<int64_t>(ti) * kNb + qi + 4 * h) * np +
                        static_cast<int64_t>(tj) * kNb + qj + l32;
         f32x16 cold[2][2], acc[2][2];
@@ -580,6 +633,30 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
                 for (int e = 0; e < 16; ++e)
                     cbase[(32 * x + 8 * (e >> 2) + (e & 3)) * np + 32 * y] = acc[x][y][e];
     }
+}
+
+__global__ void __launch_bounds__(kTrThreads)
+bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
+                    const float *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
+                    int32_t wpx, int32_t fuse, int32_t t_skip, float *__restrict__ rp_next,
+                    double *__restrict__ pt, int32_t *__restrict__ singular)
+{
+    __shared__ __attribute__((aligned(16))) GjLds lds;
+    if (fuse && blockIdx.x == 0) {
+        // tile (kp+1, kp+1) -- position t_skip of the full tile order -- by
+        // the same pipeline as every other tile, then its inverse
+        TrPlan one;
+        one.first = t_skip;
+        one.stride = 1;
+        one.n_my = 1;
+        trailing_tiles(a, ct, rp, np, nt, kp, (nt - 1) * nt + 1, one, lds.t);
+        __syncthreads();  // the updated block is visible to the workgroup
+        pivot_block(a, np, kp + 1, pt, rp_next, singular, lds.p);
+        return;
+    }
+    const int32_t n_tiles = (nt - 1) * nt - fuse;
+    trailing_tiles(a, ct, rp, np, nt, kp, fuse ? t_skip : n_tiles + 1,
+                   trailing_plan(n_tiles, wpx, fuse), lds.t);
 }
 
 // ---------------------------------------------------------------- refinement
@@ -765,7 +842,7 @@ int DenseWork::reserve(int64_t n_)
     const size_t f = sizeof(float), d = sizeof(double);
     void *p[8] = {};
     const size_t sz[8] = {static_cast<size_t>(np_ * np_) * f, static_cast<size_t>(kNb * np_) * f,
-                          static_cast<size_t>(kNb * np_) * f, static_cast<size_t>(kNb * kNb) * d,
+                          static_cast<size_t>(2 * kNb * np_) * f, static_cast<size_t>(kNb * kNb) * d,
                           static_cast<size_t>(np_) * d, static_cast<size_t>(np_) * d,
                           static_cast<size_t>(6 * np_ + 8) * d, 256};
     for (int i = 0; i < 8; ++i) {
@@ -802,29 +879,50 @@ void DenseWork::release()
     n = np = np_cap = bytes = 0;
 }
 
-// Trailing-update launch shape: tiles per XCD and persistent workgroups
-// (at most one per CU, no more than there are tiles per XCD).
-static int32_t trailing_per_xcd(int32_t nt)
+// Trailing-update launch shape: persistent workgroups per XCD (at most one
+// per CU, no more than a share of the tiles).
+static int32_t trailing_wpx(int32_t nt)
 {
-    return ((nt - 1) * nt + kXcd - 1) / kXcd;
-}
-static unsigned int trailing_grid(int32_t nt)
-{
-    return static_cast<unsigned int>(kXcd * std::min(kTrWgPerXcd, trailing_per_xcd(nt)));
+    const int32_t per_xcd = ((nt - 1) * nt + kXcd - 1) / kXcd;
+    return std::min(kTrWgPerXcd, per_xcd);
 }
 
-// One block step's three launches.
-static void bgj_step(hipStream_t s, DenseWork &w, int32_t kp)
+// Position of tile (kp+1, kp+1) in the strip order of trailing_tile.
+static int32_t trailing_pos_next_pivot(int32_t nt, int32_t kp)
+{
+    const int32_t r = kp;  // block row kp + 1 among the rows other than kp
+    const int32_t strip = r / 4, hgt = std::min(4, nt - 1 - 4 * strip);
+    return strip * 4 * nt + (kp + 1) * hgt + (r - 4 * strip);
+}
+
+// R of block step k (two buffers: step k + 1's pivot writes its R while
+// step k's trailing update still reads R_k).
+static float *rp_of(DenseWork &w, int32_t k)
+{
+    return w.rp + static_cast<int64_t>(k & 1) * kNb * w.np_cap;
+}
+
+// One block step: pivot (unless the previous step's trailing launch already
+// made it), panels, trailing update -- which, when there is a next step and
+// at least two workgroups per XCD, also makes the next pivot.  Returns
+// whether it did.
+static bool bgj_step(hipStream_t s, DenseWork &w, int32_t kp, bool pivot_done)
 {
     const int32_t nt = static_cast<int32_t>(w.np / kNb);
-    hipLaunchKernelGGL(bgj_pivot_kernel, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp, w.pt,
-                       w.rp, w.flag);
-    if (nt == 1) return;
+    float *rpk = rp_of(w, kp);
+    if (!pivot_done)
+        hipLaunchKernelGGL(bgj_pivot_kernel, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp, w.pt,
+                           rpk, w.flag);
+    if (nt == 1) return false;
     const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
     hipLaunchKernelGGL(bgj_panel_kernel, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp, w.pt,
-                       w.rp, w.ct);
-    hipLaunchKernelGGL(bgj_trailing_kernel, dim3(trailing_grid(nt)), dim3(kTrThreads), 0, s, w.a,
-                       w.ct, w.rp, w.np, nt, kp, trailing_per_xcd(nt));
+                       rpk, w.ct);
+    const int32_t wpx = trailing_wpx(nt);
+    const int32_t fuse = (kp + 1 < nt && wpx >= 2) ? 1 : 0;
+    hipLaunchKernelGGL(bgj_trailing_kernel, dim3(kXcd * wpx), dim3(kTrThreads), 0, s, w.a, w.ct,
+                       rpk, w.np, nt, kp, wpx, fuse, fuse ? trailing_pos_next_pivot(nt, kp) : 0,
+                       rp_of(w, kp + 1), w.pt, w.flag);
+    return fuse != 0;
 }
 
 // In-place FP32 blocked inverse of w.a (np x np, padded).  *singular set on
@@ -833,7 +931,8 @@ int bgj_invert(hipStream_t s, DenseWork &w)
 {
     GSLAM_HIP(hipMemsetAsync(w.flag, 0, sizeof(int32_t), s));
     const int32_t nt = static_cast<int32_t>(w.np / kNb);
-    for (int32_t kp = 0; kp < nt; ++kp) bgj_step(s, w, kp);
+    bool pivot_done = false;
+    for (int32_t kp = 0; kp < nt; ++kp) pivot_done = bgj_step(s, w, kp, pivot_done);
     GSLAM_HIP(hipGetLastError());
     return GSLAM_OK;
 }
@@ -846,9 +945,10 @@ int bgj_time_trailing(hipStream_t s, DenseWork &w, int reps, hipEvent_t e0, hipE
         set_error("trailing update timing needs 3N > 128");
         return GSLAM_ERR_STATE;
     }
+    const int32_t wpx = trailing_wpx(nt);
     auto launch = [&]() {
-        hipLaunchKernelGGL(bgj_trailing_kernel, dim3(trailing_grid(nt)), dim3(kTrThreads), 0, s,
-                           w.a, w.ct, w.rp, w.np, nt, 0, trailing_per_xcd(nt));
+        hipLaunchKernelGGL(bgj_trailing_kernel, dim3(kXcd * wpx), dim3(kTrThreads), 0, s, w.a, w.ct,
+                           w.rp, w.np, nt, 0, wpx, 0, 0, w.rp, w.pt, w.flag);
     };
     for (int i = 0; i < 3; ++i) launch();
     GSLAM_HIP(hipGetLastError());
