@@ -23,8 +23,9 @@ Also reported:
   committed rocprofv3 counter summary when present;
 * cpu_baseline: the CPU oracle (FP64, one thread) on a bounded sample of the
   same GN step, extrapolated to the FP64 solve's iteration count;
-* gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) against
-  the plain FP64 Gauss-Jordan and numpy.linalg.solve (FP64), grid-city N=1600;
+* gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) and in
+  FP64 (GJ_BLOCKED64) against the plain FP64 Gauss-Jordan and
+  numpy.linalg.solve (FP64), grid-city N=1600;
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -200,6 +201,14 @@ def gj_section(args, gslam, G, np):
                                                factor_tflops=2.0 * n ** 3 / st["ms_factor"] / 1e9)
                 log(f"gj_mixed {name} N={N}: {st['ms']:.2f} ms (factor {st['ms_factor']:.2f}), "
                     f"{st['iters']} refinement its, err vs numpy {e:.2e}")
+            g.solve("gj64_blocked")   # warm-up (work space allocation)
+            st = g.solve("gj64_blocked")
+            e = float(np.linalg.norm(g.dx() - x_np) / nx)
+            row["gj64_blocked"] = dict(ms=st["ms"], factor_ms=st["ms_factor"], rel_dx_err_vs_numpy=e,
+                                       tflops=2.0 * n ** 3 / st["ms_factor"] / 1e9,
+                                       frac_fp64_peak=2.0 * n ** 3 / st["ms_factor"] / 1e9 / FP64_PEAK)
+            log(f"gj64_blocked N={N}: {st['ms']:.2f} ms (factor {st['ms_factor']:.2f}), "
+                f"err vs numpy {e:.2e}")
             us = g.time_kernel("gj_trailing", 10)
             npd = -(-n // 128) * 128
             fl = 2.0 * (npd - 128) * npd * 128

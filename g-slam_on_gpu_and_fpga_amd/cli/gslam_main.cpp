@@ -13,7 +13,7 @@
 // (src/gpu_pcg_fp64/GSLAM.cu:165-171) the diff is always ||dx||/sqrt(dim).
 //
 // Extra knobs (environment, defaults reproduce the reference):
-//   GSLAM_SOLVER   pcg64 | mixed | gj64 | gjmixed  (pcg64)
+//   GSLAM_SOLVER   pcg64 | mixed | gj64 | gj64b | gjmixed  (pcg64)
 //   GSLAM_TOL      PCG relative tolerance      (1e-6, MatrixOps.c:345)
 //   GSLAM_REFINE   gjmixed refinement: ir | cg (ir)
 //   GSLAM_SWEEPS   gjmixed refinement iterations (3)
@@ -163,6 +163,7 @@ int main(int argc, char *argv[])
     if (solver && !std::strcmp(solver, "mixed")) opts.solver = GSLAM_SOLVER_PCG_MIXED;
     if (solver && !std::strcmp(solver, "gj64")) opts.solver = GSLAM_SOLVER_GJ_FP64;
     if (solver && !std::strcmp(solver, "gjmixed")) opts.solver = GSLAM_SOLVER_GJ_MIXED;
+    if (solver && !std::strcmp(solver, "gj64b")) opts.solver = GSLAM_SOLVER_GJ_BLOCKED64;
     const char *refine = std::getenv("GSLAM_REFINE");
     if (refine && !std::strcmp(refine, "cg")) opts.refine_mode = GSLAM_REFINE_CG;
     opts.refine_sweeps = env_int("GSLAM_SWEEPS", opts.refine_sweeps);

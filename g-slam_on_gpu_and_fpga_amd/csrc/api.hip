@@ -235,7 +235,7 @@ int gslam_get_info(const gslam_graph *gh, gslam_graph_info *info)
     info->n_between_edges = h.n_bet_e;
     info->n_blocks = h.n_blocks;
     info->nnz = h.nnz;
-    info->device_bytes = gh->g.bytes + gh->g.dense.bytes;
+    info->device_bytes = gh->g.bytes + gh->g.dense.bytes + gh->g.dense64.bytes;
     return GSLAM_OK;
 }
 
@@ -367,7 +367,7 @@ int gslam_get_system_bsr(gslam_graph *gh, int32_t *brow_ptr, int32_t *bcol, doub
 static int check_opts(const gslam_solve_opts *o)
 {
     if (!o) return GSLAM_ERR_ARG;
-    if (o->solver < GSLAM_SOLVER_PCG_FP64 || o->solver > GSLAM_SOLVER_GJ_MIXED) {
+    if (o->solver < GSLAM_SOLVER_PCG_FP64 || o->solver > GSLAM_SOLVER_GJ_BLOCKED64) {
         set_error("unknown solver id");
         return GSLAM_ERR_ARG;
     }
@@ -387,7 +387,8 @@ static int solve_into(DeviceGraph &g, const gslam_solve_opts &o, const double *b
                       gslam_solve_stats &s)
 {
     std::memset(&s, 0, sizeof(s));
-    if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED)
+    if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED ||
+        o.solver == GSLAM_SOLVER_GJ_BLOCKED64)
         return gslam::dense_solve(g, o, b, x, s);
     if (o.solver == GSLAM_SOLVER_PCG_MIXED && !g.factors_valid) {
         set_error("mixed PCG needs the edge factors of gslam_assemble (system was set from CSR)");

@@ -145,6 +145,13 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
         }
         return dense_solve_mixed(g, o, b, x, st);
     }
+    if (o.solver == GSLAM_SOLVER_GJ_BLOCKED64) {
+        if (n > 92160) {
+            set_error("GJ_BLOCKED64 limited to 3N <= 92160 (68 GB FP64 matrix)");
+            return GSLAM_ERR_TOO_LARGE;
+        }
+        return dense_solve_blocked64(g, o, b, x, st);
+    }
     if (n > 60000) {
         set_error("dense solve limited to 3N <= 60000 (28.8 GB FP64 matrix)");
         return GSLAM_ERR_TOO_LARGE;
@@ -198,8 +205,9 @@ extern "C" int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t 
     using namespace gslam;
     if (!a || !inv || n <= 0) return GSLAM_ERR_ARG;
     if (solver == GSLAM_SOLVER_GJ_MIXED) return bgj_invert_host(a, inv, n);
+    if (solver == GSLAM_SOLVER_GJ_BLOCKED64) return bgj64_invert_host(a, inv, n);
     if (solver != GSLAM_SOLVER_GJ_FP64) {
-        set_error("gslam_gj_invert: solver must be GJ_FP64 or GJ_MIXED");
+        set_error("gslam_gj_invert: solver must be GJ_FP64, GJ_BLOCKED64 or GJ_MIXED");
         return GSLAM_ERR_ARG;
     }
     const int64_t nn = static_cast<int64_t>(n) * n;

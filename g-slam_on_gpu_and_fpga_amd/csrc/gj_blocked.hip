@@ -221,9 +221,12 @@ struct PivLds {
     double rnext[kPb][kPb];      // wave 4: (P8 A[K, K+1])
 };
 
+// T is the storage type of the matrix and of R (float for GJ_MIXED, double
+// for GJ_BLOCKED64); the block itself is inverted in FP64 either way.
 // Waves beyond the fifth take part in the barriers only.
-__device__ __attribute__((always_inline)) inline void pivot_block(float *__restrict__ a, int64_t np, int32_t kp,
-                                   double *__restrict__ pt, float *__restrict__ rp,
+template <class T>
+__device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict__ a, int64_t np, int32_t kp,
+                                   double *__restrict__ pt, T *__restrict__ rp,
                                    int32_t *__restrict__ singular, PivLds &L)
 {
     auto &colraw = L.colraw;
@@ -357,18 +360,19 @@ __device__ __attribute__((always_inline)) inline void pivot_block(float *__restr
                 const int64_t row = 32 * wave + 16 * x + h + 4 * e;
                 const int64_t col = 16 * y + l16;
                 const double v = acc[x][y][e];
-                a[(k0 + row) * np + k0 + col] = static_cast<float>(v);
-                rp[row * np + k0 + col] = static_cast<float>(v);
+                a[(k0 + row) * np + k0 + col] = static_cast<T>(v);
+                rp[row * np + k0 + col] = static_cast<T>(v);
                 pt[col * kNb + row] = v;
             }
 }
 
+template <class T>
 __global__ void __launch_bounds__(kPivThreads)
-bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
-                 float *__restrict__ rp, int32_t *__restrict__ singular)
+bgj_pivot_kernel(T *__restrict__ a, int64_t np, int32_t kp, double *__restrict__ pt,
+                 T *__restrict__ rp, int32_t *__restrict__ singular)
 {
     __shared__ PivLds lds;
-    pivot_block(a, np, kp, pt, rp, singular, lds);
+    pivot_block<T>(a, np, kp, pt, rp, singular, lds);
 }
 
 // Row panel and column panel of block step kp.
@@ -377,9 +381,10 @@ bgj_pivot_kernel(float *__restrict__ a, int64_t np, int32_t kp, double *__restri
 //    and to row panel K of A (FP32); column tiles inside panel K skip.
 //  otherwise: CT[:, i0:i0+64] = -A[i0:i0+64, K]^T through an LDS transpose,
 //    row tiles inside panel K skip.
+template <class T>
 __global__ void __launch_bounds__(kThreads)
-bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__restrict__ pt,
-                 float *__restrict__ rp, float *__restrict__ ct)
+bgj_panel_kernel(T *__restrict__ a, int64_t np, int32_t kp, const double *__restrict__ pt,
+                 T *__restrict__ rp, T *__restrict__ ct)
 {
     const int64_t k0 = static_cast<int64_t>(kp) * kNb;
     const int32_t n_col_tiles = static_cast<int32_t>(np / 32);
@@ -394,7 +399,7 @@ bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__
 #pragma unroll
             for (int y = 0; y < 2; ++y) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
         const double *pa = pt + static_cast<int64_t>(h) * kNb + 32 * wave + l16;
-        const float *pb = a + (k0 + h) * np + j0 + l16;
+        const T *pb = a + (k0 + h) * np + j0 + l16;
 #pragma unroll 4
         for (int s = 0; s < kNb / 4; ++s) {
             const double a0 = pa[4 * s * kNb], a1 = pa[4 * s * kNb + 16];
@@ -416,13 +421,13 @@ bgj_panel_kernel(float *__restrict__ a, int64_t np, int32_t kp, const double *__
                 for (int e = 0; e < 4; ++e) {
                     const int64_t row = 32 * wave + 16 * x + h + 4 * e;
                     const int64_t col = j0 + 16 * y + l16;
-                    const float v = static_cast<float>(acc[x][y][e]);
+                    const T v = static_cast<T>(acc[x][y][e]);
                     rp[row * np + col] = v;
                     a[(k0 + row) * np + col] = v;
                 }
         return;
     }
-    __shared__ float tile[kNb][65];
+    __shared__ T tile[kNb][65];
     const int64_t i0 = 64 * static_cast<int64_t>(blockIdx.x - n_col_tiles);
     if (i0 >= np || i0 / kNb == kp) return;
 #pragma unroll 4
@@ -651,12 +656,141 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
         one.n_my = 1;
         trailing_tiles(a, ct, rp, np, nt, kp, (nt - 1) * nt + 1, one, lds.t);
         __syncthreads();  // the updated block is visible to the workgroup
-        pivot_block(a, np, kp + 1, pt, rp_next, singular, lds.p);
+        pivot_block<float>(a, np, kp + 1, pt, rp_next, singular, lds.p);
         return;
     }
     const int32_t n_tiles = (nt - 1) * nt - fuse;
     trailing_tiles(a, ct, rp, np, nt, kp, fuse ? t_skip : n_tiles + 1,
                    trailing_plan(n_tiles, wpx, fuse), lds.t);
+}
+
+// ---------------------------------------------------------------- FP64 blocked
+
+// GJ_BLOCKED64 trailing update of block step kp, FP64 throughout: every 128 x
+// 128 tile (ti, tj), ti != kp, A_tile = (tj == kp ? 0 : A_tile) + CT^T R,
+// one tile per workgroup of four waves (a 64 x 64 quadrant each, 4 x 4 FP64
+// MFMA 16x16x4 blocks accumulated onto the old tile, operands read as 128-byte
+// rows from L2).  An FP64 MFMA takes about four times as long per tile as
+// the tile's own HBM traffic, so the several workgroups resident on a CU keep
+// its matrix pipes busy without staging through LDS.
+__global__ void __launch_bounds__(kThreads)
+bgj_trailing64_kernel(double *__restrict__ a, const double *__restrict__ ct,
+                      const double *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
+                      int32_t per_xcd)
+{
+    const int32_t w = blockIdx.x;
+    const int32_t t = (w % kXcd) * per_xcd + w / kXcd;
+    if (t >= (nt - 1) * nt) return;
+    int32_t ti, tj;
+    trailing_tile(t, nt, kp, ti, tj);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int l16 = lane & 15, h = lane >> 4;
+    const int64_t i0 = static_cast<int64_t>(ti) * kNb + 64 * (wave >> 1);
+    const int64_t j0 = static_cast<int64_t>(tj) * kNb + 64 * (wave & 1);
+    double *cbase = a + (i0 + h) * np + j0 + l16;  // (x, y, e): row 16x + 4e, col 16y
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                acc[x][y][e] = tj != kp ? cbase[(16 * x + 4 * e) * np + 16 * y] : 0.0;
+    const double *pa = ct + static_cast<int64_t>(h) * np + i0 + l16;
+    const double *pb = rp + static_cast<int64_t>(h) * np + j0 + l16;
+#pragma unroll 2
+    for (int s4 = 0; s4 < kNb / 4; ++s4) {
+        const int64_t o = 4 * static_cast<int64_t>(s4) * np;
+        double av[4], bv[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) av[x] = pa[o + 16 * x];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) bv[y] = pb[o + 16 * y];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y)
+                acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cbase[(16 * x + 4 * e) * np + 16 * y] = acc[x][y][e];
+}
+
+// a = identity on the padding, zero elsewhere (np x np FP64).
+__global__ void __launch_bounds__(kThreads)
+bgj_clear64_kernel(double *__restrict__ a, int64_t n, int64_t np)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= np * np) return;
+    const int64_t i = t / np, j = t - i * np;
+    a[t] = (i >= n && i == j) ? 1.0 : 0.0;
+}
+
+// Jacobi-scaled BSR blocks of Omega (dsc_i Omega_ij dsc_j) into the padded
+// FP64 matrix.
+__global__ void __launch_bounds__(kThreads)
+bgj_scatter_bsr64_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                         const double *__restrict__ bval, int64_t n_blocks, int32_t n_rows,
+                         const double *__restrict__ dsc, int64_t np, double *__restrict__ a)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= 9 * n_blocks) return;
+    const int64_t b = t / 9;
+    const int k = static_cast<int>(t - 9 * b);
+    int32_t lo = 0, hi = n_rows - 1;  // block row of b
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (brow_ptr[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const int64_t i = 3 * static_cast<int64_t>(lo) + k / 3;
+    const int64_t j = 3 * static_cast<int64_t>(bcol[b]) + k % 3;
+    a[i * np + j] = bval[t] * dsc[i] * dsc[j];
+}
+
+// n x n host-layout matrix into the padded matrix, Jacobi-scaled, and the
+// inverse back out (unscaled; zeros if the inversion met a zero pivot).
+__global__ void __launch_bounds__(kThreads)
+bgj_load_dense64_kernel(const double *__restrict__ m, int64_t n, int64_t np,
+                        const double *__restrict__ dsc, double *__restrict__ a)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= np * np) return;
+    const int64_t i = t / np, j = t - i * np;
+    a[t] = (i < n && j < n) ? m[i * n + j] * dsc[i] * dsc[j] : (i == j ? 1.0 : 0.0);
+}
+
+__global__ void __launch_bounds__(kThreads)
+bgj_store_dense64_kernel(const double *__restrict__ a, int64_t n, int64_t np,
+                         const double *__restrict__ dsc, const int32_t *__restrict__ singular,
+                         double *__restrict__ m)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t i = t / n, j = t - i * n;
+    m[t] = *singular ? 0.0 : dsc[i] * a[i * np + j] * dsc[j];
+}
+
+// x = D A[0:n, 0:n] D b (D = diag(dsc)) with the padded row stride np, one
+// wave per row: the solve with the inverse of the scaled matrix.
+__global__ void __launch_bounds__(kThreads)
+bgj_matvec64_kernel(const double *__restrict__ a, int64_t np, int64_t n,
+                    const double *__restrict__ dsc, const double *__restrict__ b,
+                    double *__restrict__ x)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave0 = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t i = wave0; i < n; i += nwaves) {
+        const double *row = a + i * np;
+        double s = 0.0;
+        for (int64_t j = lane; j < n; j += kWave) s = fma(row[j], dsc[j] * b[j], s);
+        s = wave_sum(s);
+        if (lane == 0) x[i] = dsc[i] * s;
+    }
 }
 
 // ---------------------------------------------------------------- refinement
@@ -911,11 +1045,11 @@ static bool bgj_step(hipStream_t s, DenseWork &w, int32_t kp, bool pivot_done)
     const int32_t nt = static_cast<int32_t>(w.np / kNb);
     float *rpk = rp_of(w, kp);
     if (!pivot_done)
-        hipLaunchKernelGGL(bgj_pivot_kernel, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp, w.pt,
+        hipLaunchKernelGGL(bgj_pivot_kernel<float>, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp, w.pt,
                            rpk, w.flag);
     if (nt == 1) return false;
     const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
-    hipLaunchKernelGGL(bgj_panel_kernel, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp, w.pt,
+    hipLaunchKernelGGL(bgj_panel_kernel<float>, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp, w.pt,
                        rpk, w.ct);
     const int32_t wpx = trailing_wpx(nt);
     const int32_t fuse = (kp + 1 < nt && wpx >= 2) ? 1 : 0;
@@ -1107,6 +1241,170 @@ int bgj_invert_host(const double *m_host, double *inv_host, int64_t n)
         }
         if (rc == GSLAM_OK) {
             hipLaunchKernelGGL(bgj_store_dense_kernel, dim3(nblk(n * n)), dim3(kThreads), 0, s, w.a,
+                               n, np, w.dsc, w.flag, m);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(inv_host, m, static_cast<size_t>(n * n) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipMemcpyAsync(&singular, w.flag, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                set_error("gslam_gj_invert: HIP error");
+                rc = GSLAM_ERR_HIP;
+            }
+        }
+    }
+    (void)hipFree(m);
+    w.release();
+    (void)hipStreamDestroy(s);
+    if (rc == GSLAM_OK && singular) {
+        set_error("Gauss-Jordan: zero pivot (the inverse is the zero matrix)");
+        return GSLAM_ERR_SINGULAR;
+    }
+    return rc;
+}
+
+
+// ---------------------------------------------------------------- FP64 blocked, host
+
+int DenseWork64::reserve(int64_t n_)
+{
+    const int64_t np_ = bgj_padded(n_);
+    if (a && np_ <= np_cap) {
+        n = n_;
+        np = np_;
+        return GSLAM_OK;
+    }
+    release();
+    const size_t d = sizeof(double);
+    void *p[6] = {};
+    const size_t sz[6] = {static_cast<size_t>(np_ * np_) * d, static_cast<size_t>(kNb * np_) * d,
+                          static_cast<size_t>(kNb * np_) * d, static_cast<size_t>(kNb * kNb) * d,
+                          static_cast<size_t>(np_) * d, 256};
+    for (int i = 0; i < 6; ++i) {
+        if (hipMalloc(&p[i], sz[i]) != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
+            set_error("blocked FP64 Gauss-Jordan: out of device memory for a " +
+                      std::to_string(np_) + " x " + std::to_string(np_) + " FP64 matrix");
+            return GSLAM_ERR_NOMEM;
+        }
+    }
+    a = static_cast<double *>(p[0]);
+    ct = static_cast<double *>(p[1]);
+    rp = static_cast<double *>(p[2]);
+    pt = static_cast<double *>(p[3]);
+    dsc = static_cast<double *>(p[4]);
+    flag = static_cast<int32_t *>(p[5]);
+    n = n_;
+    np = np_cap = np_;
+    bytes = 0;
+    for (size_t b : sz) bytes += static_cast<int64_t>(b);
+    return GSLAM_OK;
+}
+
+void DenseWork64::release()
+{
+    void *p[6] = {a, ct, rp, pt, dsc, flag};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    a = ct = rp = pt = dsc = nullptr;
+    flag = nullptr;
+    n = np = np_cap = bytes = 0;
+}
+
+// In-place FP64 blocked inverse of w.a (np x np, padded): per block step the
+// FP64 pivot block, the FP64 MFMA row panel and the FP64 MFMA trailing update.
+// The matrix is Jacobi-scaled (unit diagonal) like GJ_MIXED's: the pivot
+// block's folded rank-8 update (pivot_block) forms P8 from terms of the size
+// of A_KK, which costs accuracy when |A_KK| is far from 1.
+static int bgj64_invert(hipStream_t s, DenseWork64 &w)
+{
+    GSLAM_HIP(hipMemsetAsync(w.flag, 0, sizeof(int32_t), s));
+    const int32_t nt = static_cast<int32_t>(w.np / kNb);
+    const int32_t tiles = (nt - 1) * nt;
+    const int32_t per = (tiles + kXcd - 1) / kXcd;
+    const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
+    for (int32_t kp = 0; kp < nt; ++kp) {
+        hipLaunchKernelGGL(bgj_pivot_kernel<double>, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp,
+                           w.pt, w.rp, w.flag);
+        if (nt == 1) break;
+        hipLaunchKernelGGL(bgj_panel_kernel<double>, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp,
+                           w.pt, w.rp, w.ct);
+        hipLaunchKernelGGL(bgj_trailing64_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
+                           w.rp, w.np, nt, kp, per);
+    }
+    GSLAM_HIP(hipGetLastError());
+    return GSLAM_OK;
+}
+
+int dense_solve_blocked64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                          gslam_solve_stats &st)
+{
+    (void)o;
+    const int64_t n = 3 * static_cast<int64_t>(g.h.n);
+    DenseWork64 &w = g.dense64;
+    GSLAM_TRY(w.reserve(n));
+    const int64_t np = w.np;
+    hipStream_t s = g.stream;
+    GSLAM_HIP(hipEventRecord(g.ev0, s));
+    hipLaunchKernelGGL(bgj_clear64_kernel, dim3(nblk(np * np)), dim3(kThreads), 0, s, w.a, n, np);
+    hipLaunchKernelGGL(bgj_scale_bsr_kernel, dim3(nblk(np)), dim3(kThreads), 0, s, g.bdiag, g.bval,
+                       n, np, w.dsc);
+    hipLaunchKernelGGL(bgj_scatter_bsr64_kernel, dim3(nblk(9 * g.h.n_blocks)), dim3(kThreads), 0, s,
+                       g.brow_ptr, g.bcol, g.bval, g.h.n_blocks, g.h.n, w.dsc, np, w.a);
+    GSLAM_TRY(bgj64_invert(s, w));
+    GSLAM_HIP(hipEventRecord(g.ev2, s));
+    const unsigned int gm = static_cast<unsigned int>(std::min<int64_t>(kMaxGrid, (n + 3) / 4));
+    hipLaunchKernelGGL(bgj_matvec64_kernel, dim3(gm), dim3(kThreads), 0, s, w.a, np, n, w.dsc, b, x);
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev1, s));
+    int32_t singular = 0;
+    GSLAM_HIP(hipMemcpyAsync(g.scal_host + 3, w.flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GSLAM_HIP(hipStreamSynchronize(s));
+    std::memcpy(&singular, g.scal_host + 3, sizeof(int32_t));
+    float ms = 0.f, msf = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    GSLAM_HIP(hipEventElapsedTime(&msf, g.ev0, g.ev2));
+    st.ms = ms;
+    st.ms_factor = msf;
+    st.iters = 0;
+    st.converged = singular ? 0 : 1;
+    st.flags = 0;
+    st.reliable_updates = 0;
+    if (singular) {
+        set_error("Gauss-Jordan: zero pivot (the solution is the zero vector)");
+        GSLAM_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(n) * sizeof(double), s));
+        GSLAM_HIP(hipStreamSynchronize(s));
+        st.true_rel_res = st.rel_res = 1.0;
+        return GSLAM_ERR_SINGULAR;
+    }
+    GSLAM_TRY(true_rel_residual(g, b, x, &st.true_rel_res));
+    st.rel_res = st.true_rel_res;
+    return GSLAM_OK;
+}
+
+int bgj64_invert_host(const double *m_host, double *inv_host, int64_t n)
+{
+    hipStream_t s = nullptr;
+    GSLAM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DenseWork64 w;
+    double *m = nullptr;
+    int rc = w.reserve(n);
+    if (rc == GSLAM_OK && hipMalloc(reinterpret_cast<void **>(&m), static_cast<size_t>(n * n) * 8) != hipSuccess) {
+        set_error("gslam_gj_invert: out of device memory");
+        rc = GSLAM_ERR_NOMEM;
+    }
+    int32_t singular = 0;
+    if (rc == GSLAM_OK) {
+        const int64_t np = w.np;
+        if (hipMemcpyAsync(m, m_host, static_cast<size_t>(n * n) * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = GSLAM_ERR_HIP;
+        } else {
+            hipLaunchKernelGGL(bgj_scale_dense_kernel, dim3(nblk(np)), dim3(kThreads), 0, s, m, n,
+                               np, w.dsc);
+            hipLaunchKernelGGL(bgj_load_dense64_kernel, dim3(nblk(np * np)), dim3(kThreads), 0, s, m,
+                               n, np, w.dsc, w.a);
+            rc = bgj64_invert(s, w);
+        }
+        if (rc == GSLAM_OK) {
+            hipLaunchKernelGGL(bgj_store_dense64_kernel, dim3(nblk(n * n)), dim3(kThreads), 0, s, w.a,
                                n, np, w.dsc, w.flag, m);
             if (hipGetLastError() != hipSuccess ||
                 hipMemcpyAsync(inv_host, m, static_cast<size_t>(n * n) * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -58,6 +58,18 @@ struct DenseWork {
     ~DenseWork() { release(); }
 };
 
+// GJ_BLOCKED64 work space: the padded, Jacobi-scaled FP64 matrix (inverted
+// in place), the FP64 column and row panels, P^T of the pivot block, the
+// scale factors and a zero-pivot flag.
+struct DenseWork64 {
+    double *a = nullptr, *ct = nullptr, *rp = nullptr, *pt = nullptr, *dsc = nullptr;
+    int32_t *flag = nullptr;
+    int64_t n = 0, np = 0, np_cap = 0, bytes = 0;
+    int reserve(int64_t n_);
+    void release();
+    ~DenseWork64() { release(); }
+};
+
 struct DeviceGraph {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -114,6 +126,7 @@ struct DeviceGraph {
     float *frec = nullptr;
 
     DenseWork dense;             // GJ_MIXED work space (allocated on first use)
+    DenseWork64 dense64;         // GJ_BLOCKED64 work space (allocated on first use)
     std::vector<hipEvent_t> prof_ev;  // in-solve kernel timing (created on first use)
 
     std::vector<void *> allocs;
@@ -165,6 +178,9 @@ int dense_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b
                       gslam_solve_stats &s);
 // FP32 blocked inverse of a host n x n FP64 matrix, returned as FP64.
 int bgj_invert_host(const double *m_host, double *inv_host, int64_t n);
+int dense_solve_blocked64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
+                          gslam_solve_stats &st);
+int bgj64_invert_host(const double *m_host, double *inv_host, int64_t n);
 // Mean duration of the trailing-update kernel of block step 0 on w.
 int bgj_time_trailing(hipStream_t s, DenseWork &w, int reps, hipEvent_t e0, hipEvent_t e1,
                       double *us);

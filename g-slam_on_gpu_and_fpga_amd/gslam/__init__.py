@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("GSLAM_LIB", os.path.join(PKG_DIR, "libgslam.so"))
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_STATE, ERR_TOO_LARGE, ERR_NOT_CONVERGED = (
     -1, -2, -3, -4, -5, -6, -7)
-SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3}
+SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3, "gj64_blocked": 4}
 KERNELS = {"spmv64": 0, "update64": 1, "spmv_mixed": 2, "assembly": 3, "gj_trailing": 4}
 REFINE = {"ir": 0, "cg": 1}
 
@@ -392,8 +392,9 @@ class Graph:
 def gj_invert(a, solver="gj64") -> np.ndarray:
     """Dense unpivoted Gauss-Jordan inverse on the GPU (host in, host out).
 
-    "gj64": FP64, one pass per pivot; "gj_mixed": blocked, FP32 MFMA
-    trailing updates on the Jacobi-scaled matrix (FP32 accuracy)."""
+    "gj64": FP64, one pass per pivot; "gj64_blocked": FP64, 128 pivots per
+    block step with FP64 MFMA trailing updates; "gj_mixed": blocked, FP32
+    MFMA trailing updates on the Jacobi-scaled matrix (FP32 accuracy)."""
     a = np.ascontiguousarray(a, np.float64)
     n = a.shape[0]
     out = np.zeros_like(a)

@@ -102,6 +102,8 @@ typedef struct {
 #define GSLAM_SOLVER_GJ_FP64    2  /* dense unpivoted Gauss-Jordan, FP64     */
 #define GSLAM_SOLVER_GJ_MIXED   3  /* blocked GJ, FP32 MFMA trailing updates,
                                       FP64 iterative refinement             */
+#define GSLAM_SOLVER_GJ_BLOCKED64 4 /* blocked GJ in FP64 (FP64 MFMA trailing
+                                      updates; same pivots as GJ_FP64)      */
 
 typedef struct {
     int32_t solver;             /* GSLAM_SOLVER_*                            */
@@ -206,8 +208,9 @@ int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
 
 /* Dense inverse of a host matrix by unpivoted Gauss-Jordan
  * (src/cpu_pcg_fp64/MatrixOps.c:29-91; GPU twin src/gpu_gj_fp64/inv.cu).
- * solver: GSLAM_SOLVER_GJ_FP64 or GSLAM_SOLVER_GJ_MIXED (FP32 MFMA blocked,
- * no refinement of the inverse itself).  GSLAM_ERR_SINGULAR on a zero pivot
+ * solver: GSLAM_SOLVER_GJ_FP64, GSLAM_SOLVER_GJ_BLOCKED64 (FP64, 128 pivots
+ * per block step) or GSLAM_SOLVER_GJ_MIXED (FP32 MFMA blocked, no refinement
+ * of the inverse itself).  GSLAM_ERR_SINGULAR on a zero pivot
  * (inv is then all zeros, as in the reference). */
 int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
 
