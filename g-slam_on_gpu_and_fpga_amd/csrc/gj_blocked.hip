@@ -1,6 +1,7 @@
-// Blocked Gauss-Jordan inverse in FP32 with MFMA trailing updates, and the
-// GJ_MIXED dense solve: FP32 inverse of the Jacobi-scaled Omega, then FP64
-// refinement on the true residual xi - Omega dx.
+// Blocked Gauss-Jordan inverse: the GJ_MIXED dense solve (FP32 inverse of the
+// Jacobi-scaled Omega with FP32 MFMA trailing updates, then FP64 refinement
+// on the true residual xi - Omega dx) and GJ_BLOCKED64 (the same block steps
+// in FP64 with FP64 MFMA trailing updates, no refinement).
 //
 // The reference inverts Omega with an unblocked, unpivoted Gauss-Jordan
 // (inv, src/cpu_pcg_fp64/MatrixOps.c:29-91; inv_float,
@@ -8,22 +9,25 @@
 // 221-354): one pass over the whole matrix per pivot.  Here the pivots are
 // taken kNb = 128 at a time (still unpivoted, in place):
 //
-//   P    = A_KK^-1                      FP64, one workgroup (pivot kernel)
-//   R_J  = P A_KJ            (J != K)   FP64 MFMA, stored FP32 (row panel)
+//   P    = A_KK^-1                      FP64, one workgroup (pivot_block)
+//   R_J  = P A_KJ            (J != K)   FP64 MFMA (row panel)
 //   CT   = -A_IK^T           (I != K)   transposed copy of the column panel
-//   A_IJ = A_IJ + CT^T R_J   (I, J != K)  FP32 MFMA 32x32x2, FP32 accumulate
+//   A_IJ = A_IJ + CT^T R_J   (I, J != K)  MFMA trailing update
 //   A_IK = CT^T P            (I != K)   same kernel, R_K = P
 //   A_KJ = R_J, A_KK = P
 //
 // which is 2 n^3 flops in total, almost all in the trailing update.  The
 // matrix is padded to a multiple of kNb with an identity block (its inverse
 // is the identity, and exact zeros stay exact, so the padding never mixes
-// into the result).
+// into the result).  In GJ_MIXED the trailing launch of step K also updates
+// and inverts the pivot block of step K+1 (its workgroup 0), so only the
+// first pivot block has a launch of its own.
 //
 // Precision: Omega is scaled to S = D^-1/2 Omega D^-1/2 (unit diagonal) in
-// FP64 before rounding to FP32; the pivot blocks are inverted and the pivot
-// rows scaled in FP64; the trailing GEMMs run in FP32.  The FP32 inverse M
-// then preconditions FP64 refinement on Omega itself:
+// FP64 before rounding to the storage type; the pivot blocks are inverted
+// and the pivot rows scaled in FP64; GJ_MIXED stores and updates in FP32,
+// GJ_BLOCKED64 in FP64.  The FP32 inverse M preconditions FP64 refinement
+// on Omega itself:
 //   GSLAM_REFINE_IR: x += D^-1/2 M D^-1/2 (xi - Omega x), refine_sweeps times;
 //   GSLAM_REFINE_CG: conjugate gradients preconditioned by the same operator,
 //     flexible (Polak-Ribiere) beta because M is not exactly symmetric,
