@@ -121,6 +121,24 @@ def test_cli_drop_in(tmp_path):
     assert bad.returncode == 1 and bad.stdout.startswith("Usage: ")
 
 
+def test_cli_dense_solvers_agree(tmp_path):
+    """GSLAM_SOLVER=gj64b (blocked FP64) against gj64 (one pivot per launch):
+    the same unpivoted elimination, so the same poses after 3 rounds to the
+    1e-6 m/rad of the other GN comparisons (kappa ~ 1e9 on this system)."""
+    path = data_path("cityTrees100.log.txt")
+    dumps = {}
+    for solver in ("gj64", "gj64b"):
+        dumps[solver] = tmp_path / f"{solver}.txt"
+        out = subprocess.run([GSLAM_BIN, "1", "0.1", "3", "1", path], capture_output=True,
+                             text=True, timeout=120, check=True,
+                             env=dict(os.environ, POSE_DUMP_FILE=str(dumps[solver]),
+                                      GSLAM_SOLVER=solver))
+        assert len(out.stdout.splitlines()) == 5
+    a, b = np.loadtxt(dumps["gj64"]), np.loadtxt(dumps["gj64b"])
+    assert np.array_equal(a[:, 0], b[:, 0])
+    assert np.abs(a[:, 1:] - b[:, 1:]).max() <= 1e-6
+
+
 class _U(C.Structure):
     _fields_ = [("step", C.c_uint), ("nu", C.c_double), ("omega", C.c_double)]
 
