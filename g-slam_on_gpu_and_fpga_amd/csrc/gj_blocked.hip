@@ -49,7 +49,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNb = 128;          // panel width, trailing tile edge and GEMM depth
 constexpr int kPb = 8;            // pivot kernel: pivots per inner block step
-constexpr int kPivThreads = 5 * 64;  // pivot kernel: four MFMA waves and one pivot wave
+constexpr int kPivThreads = 8 * 64;  // pivot kernel: six MFMA waves, one pivot-chain wave
 constexpr int kXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
 
 // ---------------------------------------------------------------- loading
@@ -200,21 +200,27 @@ __device__ inline void static_for(std::integer_sequence<int, I...>, F &&f)
 }
 
 // P = A_KK^-1 (128 x 128) in FP64 by Gauss-Jordan in blocks of 8 pivots,
-// one workgroup of five waves.  Waves 0-3 hold the block in FP64 MFMA
-// accumulator layout (wave w: rows 32w..32w+31, 2 x 8 tiles of 16 x 16;
+// one workgroup of eight waves.  The MFMA waves hold the block in FP64 MFMA
+// accumulator layout as 16-row tiles (one or two per wave, 8 x 16 x 16 each;
 // register e of lane l in a tile holds (l/16 + 4e, l%16)).  Step pb
-// (pivots 8pb..8pb+7, "K"; P8 = A_KK^-1 made by wave 4 one step ahead):
-//   1. waves 0-3 publish column panel A[:, K], row panel A[K, :] with I
+// (pivots 8pb..8pb+7, "K"; P8 = A_KK^-1 made by the chain wave one step
+// ahead):
+//   1. the MFMA waves publish column panel A[:, K], row panel A[K, :] with I
 //      added to its A_KK part, and the next pivot block to LDS; barrier;
-//   2. waves 0-3: A += C B with FP64 MFMA 16x16x4 (rank 8), where
+//   2. the MFMA waves: A += C B with FP64 MFMA 16x16x4 (rank 8), where
 //      C[i, :] = -A[i, K] P8 (i not in K), C[K, :] = P8 - I and
 //      B = A[K, :] + [0 I 0], which in one update gives the trailing
 //      entries A_ij - A_iK P8 A_Kj, row K = P8 A_Kj, column K = -A_iK P8 and
-//      block KK = P8; meanwhile wave 4 applies the step to the next pivot
-//      block and inverts it (gj8_wave), so the serial pivot chain runs
+//      block KK = P8; meanwhile the chain wave applies the step to the next
+//      pivot block and inverts it (gj8_wave), so the serial pivot chain runs
 //      beside the MFMAs.
+// Roles by wave: a CU places wave w of a workgroup on SIMD w % 4, and FP64
+// MFMA and FP64 VALU work on one SIMD do not overlap, so the chain wave (3)
+// has SIMD 3 to itself while waves 0, 1 (two tiles each), 2, 4, 5, 6 (one
+// each) keep 3 / 3 / 2 tiles on SIMDs 0 / 1 / 2.  Wave 7 and waves beyond the
+// eighth take part in the barriers only.
 // LDS buffers alternate with the parity of pb: one barrier per step.  Writes
-// A_KK = P and R_K = P (FP32) and PT = P^T (FP64) for the row-panel kernel;
+// A_KK = P and R_K = P (in T) and PT = P^T (FP64) for the row-panel kernel;
 // a zero pivot sets *singular (the whole inverse is then reported as zero,
 // MatrixOps.c:60-68).
 struct PivLds {
@@ -222,12 +228,18 @@ struct PivLds {
     double rowraw[2][kPb][kNb];  // A[K, :] + [0 I 0]
     double pblk[2][kPb][kPb];    // P8 of step pb (parity pb)
     double nextblk[2][kPb][kPb]; // A[K+1, K+1] before step pb
-    double rnext[kPb][kPb];      // wave 4: (P8 A[K, K+1])
+    double rnext[kPb][kPb];      // chain wave: (P8 A[K, K+1])
 };
+
+constexpr int kChainWave = 3;
+// 16-row tiles of wave w: count and first tile; owner wave and slot of tile rt
+__device__ constexpr int piv_tiles(int w) { return w < 2 ? 2 : (w == 2 || (w >= 4 && w <= 6)) ? 1 : 0; }
+__device__ constexpr int piv_first_tile(int w) { return w < 3 ? 2 * w : w + 1; }
+__device__ constexpr int piv_owner(int rt) { return rt < 4 ? rt >> 1 : (rt == 4 ? 2 : rt - 1); }
+__device__ constexpr int piv_slot(int rt) { return rt < 4 ? (rt & 1) : 0; }
 
 // T is the storage type of the matrix and of R (float for GJ_MIXED, double
 // for GJ_BLOCKED64); the block itself is inverted in FP64 either way.
-// Waves beyond the fifth take part in the barriers only.
 template <class T>
 __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict__ a, int64_t np, int32_t kp,
                                    double *__restrict__ pt, T *__restrict__ rp,
@@ -241,21 +253,25 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
     constexpr int kSteps = kNb / kPb;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int l16 = lane & 15, h = lane >> 4;
-    const int pi = lane >> 3, pj = lane & 7;  // wave 4: entry of the 8 x 8 block
+    const int pi = lane >> 3, pj = lane & 7;  // chain wave: entry of the 8 x 8 block
     const int64_t k0 = static_cast<int64_t>(kp) * kNb;
+    const int ntile = wave < 8 ? piv_tiles(wave) : 0;
+    const int rt0 = piv_first_tile(wave);
     f64x4 acc[2][8];
     bool zero = false;
-    if (wave < 4) {
+    if (ntile > 0) {
 #pragma unroll
         for (int x = 0; x < 2; ++x)
+            if (x < ntile) {
 #pragma unroll
-            for (int y = 0; y < 8; ++y)
+                for (int y = 0; y < 8; ++y)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int64_t row = 32 * wave + 16 * x + h + 4 * e;
-                    acc[x][y][e] = static_cast<double>(a[(k0 + row) * np + k0 + 16 * y + l16]);
-                }
-    } else if (wave == 4) {
+                    for (int e = 0; e < 4; ++e) {
+                        const int64_t row = 16 * (rt0 + x) + h + 4 * e;
+                        acc[x][y][e] = static_cast<double>(a[(k0 + row) * np + k0 + 16 * y + l16]);
+                    }
+            }
+    } else if (wave == kChainWave) {
         const double v = static_cast<double>(a[(k0 + pi) * np + k0 + pj]);
         pblk[0][pi][pj] = gj8_wave(v, pi, pj, zero);
     }
@@ -263,17 +279,19 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
     static_for(std::make_integer_sequence<int, kSteps>{}, [&](auto pbc) {
         constexpr int pb = decltype(pbc)::value;
         constexpr int buf = pb & 1;
-        constexpr int yk = pb >> 1, half = pb & 1;       // column tile / lane half of K
-        constexpr int xk = (pb >> 1) & 1, wk = pb >> 2;  // row tile / wave of K
+        constexpr int yk = pb >> 1, half = pb & 1;  // column tile / lane half of K
+        constexpr int wk = piv_owner(pb >> 1), xk = piv_slot(pb >> 1);  // row tile of K
         const bool lane_in_k = (l16 >> 3) == half;
         // 1. publish
-        if (wave < 4) {
+        if (ntile > 0) {
             if (lane_in_k) {
 #pragma unroll
                 for (int x = 0; x < 2; ++x)
+                    if (x < ntile) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        colraw[buf][32 * wave + 16 * x + h + 4 * e][l16 - 8 * half] = acc[x][yk][e];
+                        for (int e = 0; e < 4; ++e)
+                            colraw[buf][16 * (rt0 + x) + h + 4 * e][l16 - 8 * half] = acc[x][yk][e];
+                    }
             }
             if (wave == wk) {
 #pragma unroll
@@ -288,7 +306,8 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
             }
             if constexpr (pb + 1 < kSteps) {
                 constexpr int q = pb + 1;
-                constexpr int yq = q >> 1, hq = q & 1, xq = (q >> 1) & 1, wq = q >> 2;
+                constexpr int yq = q >> 1, hq = q & 1;
+                constexpr int wq = piv_owner(q >> 1), xq = piv_slot(q >> 1);
                 if (wave == wq && (l16 >> 3) == hq) {
 #pragma unroll
                     for (int ee = 0; ee < 2; ++ee)
@@ -297,8 +316,8 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
             }
         }
         __syncthreads();
-        if (wave < 4) {
-            // 2. C in the MFMA A layout (lane l: row 16x + l%16, k = 4c + l/16)
+        if (ntile > 0) {
+            // 2. C in the MFMA A layout (lane l: row 16 rt + l%16, k = 4c + l/16)
             double pcol[2][kPb];
 #pragma unroll
             for (int c = 0; c < 2; ++c)
@@ -306,21 +325,22 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
                 for (int m = 0; m < kPb; ++m) pcol[c][m] = pblk[buf][m][4 * c + h];
             double av[2][2], bv[8][2];
 #pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                const int r = 32 * wave + 16 * x + l16;
-                double cr[kPb];
+            for (int x = 0; x < 2; ++x)
+                if (x < ntile) {
+                    const int r = 16 * (rt0 + x) + l16;
+                    double cr[kPb];
 #pragma unroll
-                for (int m = 0; m < kPb; ++m) cr[m] = colraw[buf][r][m];
+                    for (int m = 0; m < kPb; ++m) cr[m] = colraw[buf][r][m];
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    double s = 0.0;
+                    for (int c = 0; c < 2; ++c) {
+                        double s = 0.0;
 #pragma unroll
-                    for (int m = 0; m < kPb; ++m) s = fma(cr[m], pcol[c][m], s);
-                    av[x][c] = -s;
-                    if (wave == wk && x == xk && lane_in_k)  // row r of K: (P8 - I)
-                        av[x][c] = pblk[buf][r & 7][4 * c + h] - ((r & 7) == 4 * c + h ? 1.0 : 0.0);
+                        for (int m = 0; m < kPb; ++m) s = fma(cr[m], pcol[c][m], s);
+                        av[x][c] = -s;
+                        if (wave == wk && x == xk && lane_in_k)  // row r of K: (P8 - I)
+                            av[x][c] = pblk[buf][r & 7][4 * c + h] - ((r & 7) == 4 * c + h ? 1.0 : 0.0);
+                    }
                 }
-            }
 #pragma unroll
             for (int y = 0; y < 8; ++y)
 #pragma unroll
@@ -329,11 +349,13 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
             for (int c = 0; c < 2; ++c)
 #pragma unroll
                 for (int x = 0; x < 2; ++x)
+                    if (x < ntile) {
 #pragma unroll
-                    for (int y = 0; y < 8; ++y)
-                        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][c], bv[y][c], acc[x][y],
-                                                                       0, 0, 0);
-        } else if (wave == 4) {
+                        for (int y = 0; y < 8; ++y)
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][c], bv[y][c],
+                                                                           acc[x][y], 0, 0, 0);
+                    }
+        } else if (wave == kChainWave) {
             if constexpr (pb + 1 < kSteps) {
                 // next pivot block: A[K+1, K+1] - A[K+1, K] P8 A[K, K+1], inverted
                 constexpr int n0 = kPb * (pb + 1);
@@ -351,23 +373,25 @@ __device__ __attribute__((always_inline)) inline void pivot_block(T *__restrict_
             }
         }
     });
-    if (wave >= 4) {
-        if (wave == 4 && zero && lane == 0) *singular = 1;
+    if (ntile == 0) {
+        if (wave == kChainWave && zero && lane == 0) *singular = 1;
         return;
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
+        if (x < ntile) {
 #pragma unroll
-        for (int y = 0; y < 8; ++y)
+            for (int y = 0; y < 8; ++y)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t row = 32 * wave + 16 * x + h + 4 * e;
-                const int64_t col = 16 * y + l16;
-                const double v = acc[x][y][e];
-                a[(k0 + row) * np + k0 + col] = static_cast<T>(v);
-                rp[row * np + k0 + col] = static_cast<T>(v);
-                pt[col * kNb + row] = v;
-            }
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t row = 16 * (rt0 + x) + h + 4 * e;
+                    const int64_t col = 16 * y + l16;
+                    const double v = acc[x][y][e];
+                    a[(k0 + row) * np + k0 + col] = static_cast<T>(v);
+                    rp[row * np + k0 + col] = static_cast<T>(v);
+                    pt[col * kNb + row] = v;
+                }
+        }
 }
 
 template <class T>
@@ -671,26 +695,22 @@ bgj_trailing_kernel(float *__restrict__ a, const float *__restrict__ ct,
 // ---------------------------------------------------------------- FP64 blocked
 
 // GJ_BLOCKED64 trailing update of block step kp, FP64 throughout: every 128 x
-// 128 tile (ti, tj), ti != kp, A_tile = (tj == kp ? 0 : A_tile) + CT^T R,
-// one tile per workgroup of four waves (a 64 x 64 quadrant each, 4 x 4 FP64
-// MFMA 16x16x4 blocks accumulated onto the old tile, operands read as 128-byte
+// 128 tile (ti, tj), ti != kp, A_tile = (tj == kp ? 0 : A_tile) + CT^T R.
+// A tile is worked by four waves (a 64 x 64 quadrant each, 4 x 4 FP64 MFMA
+// 16x16x4 blocks accumulated onto the old tile, operands read as 128-byte
 // rows from L2).  An FP64 MFMA takes about four times as long per tile as
-// the tile's own HBM traffic, so the several workgroups resident on a CU keep
-// its matrix pipes busy without staging through LDS.
-__global__ void __launch_bounds__(kThreads)
-bgj_trailing64_kernel(double *__restrict__ a, const double *__restrict__ ct,
-                      const double *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
-                      int32_t per_xcd)
+// the tile's own HBM traffic, so the two tiles in flight per CU keep its
+// matrix pipes busy without staging through LDS.
+__device__ __attribute__((always_inline)) inline void trailing64_tile(
+    double *__restrict__ a, const double *__restrict__ ct, const double *__restrict__ rp,
+    int64_t np, int32_t nt, int32_t kp, int32_t t, int quad)
 {
-    const int32_t w = blockIdx.x;
-    const int32_t t = (w % kXcd) * per_xcd + w / kXcd;
-    if (t >= (nt - 1) * nt) return;
     int32_t ti, tj;
     trailing_tile(t, nt, kp, ti, tj);
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int lane = threadIdx.x % kWave;
     const int l16 = lane & 15, h = lane >> 4;
-    const int64_t i0 = static_cast<int64_t>(ti) * kNb + 64 * (wave >> 1);
-    const int64_t j0 = static_cast<int64_t>(tj) * kNb + 64 * (wave & 1);
+    const int64_t i0 = static_cast<int64_t>(ti) * kNb + 64 * (quad >> 1);
+    const int64_t j0 = static_cast<int64_t>(tj) * kNb + 64 * (quad & 1);
     double *cbase = a + (i0 + h) * np + j0 + l16;  // (x, y, e): row 16x + 4e, col 16y
     f64x4 acc[4][4];
 #pragma unroll
@@ -722,6 +742,50 @@ bgj_trailing64_kernel(double *__restrict__ a, const double *__restrict__ ct,
         for (int y = 0; y < 4; ++y)
 #pragma unroll
             for (int e = 0; e < 4; ++e) cbase[(16 * x + 4 * e) * np + 16 * y] = acc[x][y][e];
+}
+
+// Tiles of worker `half` (waves 4 half .. 4 half + 3) of this workgroup: as
+// trailing_plan, with two workers per workgroup and, under `fuse`, both
+// workers of workgroup 0 (the pivot one) taken out of XCD 0's share.
+__device__ inline TrPlan trailing64_plan(int32_t n_tiles, int32_t wpx, int32_t fuse, int32_t half)
+{
+    const int32_t b = static_cast<int32_t>(blockIdx.x), x = b % kXcd;
+    const int32_t f2 = 2 * fuse;
+    const int32_t total = 2 * kXcd * wpx - f2;
+    const int32_t cum0 = x == 0 ? 0 : 2 * x * wpx - f2, cum1 = 2 * (x + 1) * wpx - f2;
+    const int32_t lo = static_cast<int32_t>(static_cast<int64_t>(n_tiles) * cum0 / total);
+    const int32_t hi = static_cast<int32_t>(static_cast<int64_t>(n_tiles) * cum1 / total);
+    TrPlan pl;
+    pl.stride = (x == 0) ? 2 * wpx - f2 : 2 * wpx;
+    pl.first = lo + 2 * (b / kXcd) + half - ((x == 0) ? f2 : 0);
+    pl.n_my = pl.first < hi ? (hi - pl.first + pl.stride - 1) / pl.stride : 0;
+    return pl;
+}
+
+// Persistent workgroups of eight waves, one per CU, each running two tile
+// workers (waves 0-3 and 4-7) with no barrier between them.  With `fuse`,
+// workgroup 0 instead updates tile (kp+1, kp+1) (taken out of the other
+// workers' tile order) and inverts it as the pivot block of step kp+1
+// (pivot_block, writing R_{kp+1} = P into rp_next), beside the other tiles.
+__global__ void __launch_bounds__(kTrThreads)
+bgj_trailing64_kernel(double *__restrict__ a, const double *__restrict__ ct,
+                      const double *__restrict__ rp, int64_t np, int32_t nt, int32_t kp,
+                      int32_t wpx, int32_t fuse, int32_t t_skip, double *__restrict__ rp_next,
+                      double *__restrict__ pt, int32_t *__restrict__ singular)
+{
+    __shared__ PivLds lds;
+    const int wave = threadIdx.x / kWave;
+    if (fuse && blockIdx.x == 0) {
+        if (wave < 4) trailing64_tile(a, ct, rp, np, nt, kp, t_skip, wave);
+        __syncthreads();  // the updated block is visible to the workgroup
+        pivot_block<double>(a, np, kp + 1, pt, rp_next, singular, lds);
+        return;
+    }
+    const int32_t n_tiles = (nt - 1) * nt - fuse;
+    const int32_t skip = fuse ? t_skip : n_tiles + 1;
+    const TrPlan pl = trailing64_plan(n_tiles, wpx, fuse, wave >> 2);
+    for (int32_t i = 0; i < pl.n_my; ++i)
+        trailing64_tile(a, ct, rp, np, nt, kp, skip_tile(pl.first + i * pl.stride, skip), wave & 3);
 }
 
 // a = identity on the padding, zero elsewhere (np x np FP64).
@@ -1280,7 +1344,7 @@ int DenseWork64::reserve(int64_t n_)
     const size_t d = sizeof(double);
     void *p[6] = {};
     const size_t sz[6] = {static_cast<size_t>(np_ * np_) * d, static_cast<size_t>(kNb * np_) * d,
-                          static_cast<size_t>(kNb * np_) * d, static_cast<size_t>(kNb * kNb) * d,
+                          static_cast<size_t>(2 * kNb * np_) * d, static_cast<size_t>(kNb * kNb) * d,
                           static_cast<size_t>(np_) * d, 256};
     for (int i = 0; i < 6; ++i) {
         if (hipMalloc(&p[i], sz[i]) != hipSuccess) {
@@ -1313,8 +1377,23 @@ void DenseWork64::release()
     n = np = np_cap = bytes = 0;
 }
 
+// Trailing-update launch shape of GJ_BLOCKED64: workgroups per XCD, two
+// tile workers each, at most one workgroup per CU.
+static int32_t trailing64_wpx(int32_t nt)
+{
+    const int32_t per_xcd = ((nt - 1) * nt + 2 * kXcd - 1) / (2 * kXcd);
+    return std::max(1, std::min(kTrWgPerXcd, per_xcd));
+}
+
+// R of block step k (two buffers, as rp_of).
+static double *rp64_of(DenseWork64 &w, int32_t k)
+{
+    return w.rp + static_cast<int64_t>(k & 1) * kNb * w.np_cap;
+}
+
 // In-place FP64 blocked inverse of w.a (np x np, padded): per block step the
-// FP64 pivot block, the FP64 MFMA row panel and the FP64 MFMA trailing update.
+// FP64 pivot block, the FP64 MFMA row panel and the FP64 MFMA trailing update,
+// whose workgroup 0 also makes the next step's pivot block.
 // The matrix is Jacobi-scaled (unit diagonal) like GJ_MIXED's: the pivot
 // block's folded rank-8 update (pivot_block) forms P8 from terms of the size
 // of A_KK, which costs accuracy when |A_KK| is far from 1.
@@ -1322,17 +1401,23 @@ static int bgj64_invert(hipStream_t s, DenseWork64 &w)
 {
     GSLAM_HIP(hipMemsetAsync(w.flag, 0, sizeof(int32_t), s));
     const int32_t nt = static_cast<int32_t>(w.np / kNb);
-    const int32_t tiles = (nt - 1) * nt;
-    const int32_t per = (tiles + kXcd - 1) / kXcd;
     const unsigned int gp = static_cast<unsigned int>(w.np / 32 + w.np / 64);
+    const int32_t wpx = trailing64_wpx(nt);
+    bool pivot_done = false;
     for (int32_t kp = 0; kp < nt; ++kp) {
-        hipLaunchKernelGGL(bgj_pivot_kernel<double>, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np, kp,
-                           w.pt, w.rp, w.flag);
+        double *rpk = rp64_of(w, kp);
+        if (!pivot_done)
+            hipLaunchKernelGGL(bgj_pivot_kernel<double>, dim3(1), dim3(kPivThreads), 0, s, w.a, w.np,
+                               kp, w.pt, rpk, w.flag);
         if (nt == 1) break;
         hipLaunchKernelGGL(bgj_panel_kernel<double>, dim3(gp), dim3(kThreads), 0, s, w.a, w.np, kp,
-                           w.pt, w.rp, w.ct);
-        hipLaunchKernelGGL(bgj_trailing64_kernel, dim3(per * kXcd), dim3(kThreads), 0, s, w.a, w.ct,
-                           w.rp, w.np, nt, kp, per);
+                           w.pt, rpk, w.ct);
+        const int32_t fuse = kp + 1 < nt ? 1 : 0;
+        hipLaunchKernelGGL(bgj_trailing64_kernel, dim3(kXcd * wpx), dim3(kTrThreads), 0, s, w.a,
+                           w.ct, rpk, w.np, nt, kp, wpx, fuse,
+                           fuse ? trailing_pos_next_pivot(nt, kp) : 0, rp64_of(w, kp + 1), w.pt,
+                           w.flag);
+        pivot_done = fuse != 0;
     }
     GSLAM_HIP(hipGetLastError());
     return GSLAM_OK;

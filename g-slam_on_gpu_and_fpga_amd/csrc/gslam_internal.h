@@ -59,8 +59,8 @@ struct DenseWork {
 };
 
 // GJ_BLOCKED64 work space: the padded, Jacobi-scaled FP64 matrix (inverted
-// in place), the FP64 column and row panels, P^T of the pivot block, the
-// scale factors and a zero-pivot flag.
+// in place), the FP64 column panel, two row panels (block steps k and k + 1),
+// P^T of the pivot block, the scale factors and a zero-pivot flag.
 struct DenseWork64 {
     double *a = nullptr, *ct = nullptr, *rp = nullptr, *pt = nullptr, *dsc = nullptr;
     int32_t *flag = nullptr;
