@@ -11,8 +11,9 @@
 //                  agree bit for bit), applies the stopping test, forms
 //                  p = z + beta p_old on the fly for every gathered column
 //                  (double-buffered p), q = A p, and leaves p.q partials;
-//   B(k) "update": sums the p.q partials, alpha = rz/pq, updates x, r, z for
-//                  its own 3x3 block rows and leaves r.r / r.z partials.
+//   B(k) "update": sums the p.q partials, alpha = rz/pq (rz as A(k) reduced
+//                  it), updates x, r, z for its own 3x3 block rows and leaves
+//                  r.r / r.z partials.
 // Kernel boundaries order the partials; a grid-wide reduction through one
 // atomic counter costs ~15-35 us per iteration on MI355X at 1-2k
 // workgroups, the redundant re-read of <= 512 partials ~1 us.  Only
@@ -130,8 +131,16 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     T.b1 = ti.w;
     const int32_t bb = T.b0 + static_cast<int32_t>(threadIdx.x);
     T.has_blk = bb < T.b1;
-    const int32_t orow = T.r0 + static_cast<int32_t>(threadIdx.x) / 3;
     T.has_out = static_cast<int32_t>(threadIdx.x) < 3 * (T.r1 - T.r0);
+    // the output row's range and z / p_old first, from a row clamped into the
+    // tile (unconditional, so that they overlap the block loads below rather
+    // than wait for them)
+    const int32_t orow = min(T.r0 + static_cast<int32_t>(threadIdx.x) / 3, T.r1 - 1);
+    T.ob0 = brow_ptr[orow];
+    T.ob1 = brow_ptr[orow + 1];
+    const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
+    T.zo = z[o];
+    T.po = p_old[o];
     if (T.has_blk) {
         const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
         const double *v = bval + 9 * static_cast<int64_t>(bb);
@@ -142,13 +151,6 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
             T.yz[a] = v[3 * a] * z0 + v[3 * a + 1] * z1 + v[3 * a + 2] * z2;
             T.yp[a] = v[3 * a] * q0 + v[3 * a + 1] * q1 + v[3 * a + 2] * q2;
         }
-    }
-    if (T.has_out) {
-        T.ob0 = brow_ptr[orow];
-        T.ob1 = brow_ptr[orow + 1];
-        const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
-        T.zo = z[o];
-        T.po = p_old[o];
     }
 }
 
@@ -217,17 +219,30 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
 {
     __shared__ double ys[3 * kTileBlocks];
     __shared__ double red[2][kThreads / kWave];
+    static_assert(kMaxGrid <= 2 * kThreads, "two r.r / r.z partials per thread");
     SpmvTileRegs T;
     const int32_t t0 = blockIdx.x;
-    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, bcol, bval, z, p_old, T);
-    // rr_k, rz_k in one reduction (<= kMaxGrid = 2 x kThreads partials per
-    // array: one round of loads); rz_{k-1} as reduced by A(k-1)
+    // Loaded first, so that their round trips overlap the tile's loads: the
+    // stop flag and tolerance as earlier launches left them, rz_{k-1} as
+    // A(k-1) reduced it, and the r.r_k / r.z_k partials (np_cur <= 2 x
+    // kThreads: two per thread and array, summed below in strided_sum's order)
+    const int32_t done_in = status->done;
+    const double tol_in = status->tol;
     const double rz_prev = k > 0 ? rz_hist[(k + 1) & 1] : 1.0;
+    // (unconditional loads from clamped, in-range indices: a predicated load
+    // would be waited for where its branch joins)
+    const int tid = static_cast<int>(threadIdx.x);
+    const int ia = min(tid, np_cur - 1), ib = min(tid + kThreads, np_cur - 1);
+    const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
+    const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
+    asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
+    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, bcol, bval, z, p_old, T);
+    const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     double red_v[2];
-    red_v[0] = strided_sum<kThreads, 2>(pr_cur, np_cur, 0.0);
-    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
+    red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
+    red_v[1] = (0.0 + (in_a ? rz_a : 0.0)) + (in_b ? rz_b : 0.0);
     block_allsum<2, kThreads>(red_v, red);
-    if (status->done) return;
+    if (done_in) return;
     const double rr = red_v[0], rz = red_v[1];
     double tol;
     if (k == 0) {
@@ -238,7 +253,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
             status->rnorm = sqrt(rr);
         }
     } else {
-        tol = status->tol;
+        tol = tol_in;
         if (sqrt(rr) < tol) {  // MatrixOps.c:366: checked after the update
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 status->done = 1;
@@ -263,19 +278,20 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     }
 }
 
-// B(k): alpha from the p.q partials, x/r/z update, r.r / r.z partials.
-// One thread per pose: its 21 loads (x, r, p, q, D^-1) are issued before
-// the p.q and r.z partials are loaded and reduced; D^-1 is stored
-// column-major per entry ([c][3i+a]).
+// B(k): alpha from the p.q partials and the r.z that A(k) reduced from the
+// same partials (the same bits as summing them again here, without the
+// loads), x/r/z update, r.r / r.z partials.  One thread per pose: its 21
+// loads (x, r, p, q, D^-1) are issued before the p.q partials are loaded and
+// reduced; D^-1 is stored column-major per entry ([c][3i+a]).
 __global__ void __launch_bounds__(kThreads)
 pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
                   const double *__restrict__ dinv, double *__restrict__ x,
                   double *__restrict__ r, double *__restrict__ z, int32_t n,
                   const double *__restrict__ pq_in, int32_t np_pq,
-                  const double *__restrict__ pr_cur, int32_t np_cur,
-                  double *__restrict__ pr_out, PcgStatus *status)
+                  const double *__restrict__ rz_cur, double *__restrict__ pr_out,
+                  PcgStatus *status)
 {
-    __shared__ double red[2][kThreads / kWave];
+    __shared__ double red[1][kThreads / kWave];
     const int64_t dim = 3 * static_cast<int64_t>(n);
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double xp[3], rp[3], pp[3], qp[3], di[9];
@@ -293,12 +309,13 @@ pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
 #pragma unroll
             for (int a = 0; a < 3; ++a) di[3 * c + a] = dinv[c * dim + o + a];
     };
+    const int32_t done_in = status->done;  // set by earlier launches only
     if (i < n) load(i);
-    double red_v[2];
+    const double rz = *rz_cur;
+    double red_v[1];
     red_v[0] = strided_sum<kThreads>(pq_in, np_pq, 0.0);
-    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
-    block_allsum<2, kThreads>(red_v, red);
-    if (status->done) return;
+    block_allsum<1, kThreads>(red_v, red);
+    if (done_in) return;
     if (red_v[0] == 0.0) {  // MatrixOps.c:357
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
@@ -306,7 +323,7 @@ pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
         }
         return;
     }
-    const double alpha = red_v[1] / red_v[0];
+    const double alpha = rz / red_v[0];
     double v[2] = {0.0, 0.0};
     for (; i < n; i += gridDim.x * blockDim.x) {
         if (i != static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x)) load(i);
@@ -506,7 +523,7 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
             };
             auto update_args = [&](auto launch) {
                 launch(pcg_update_kernel, pnew, g.q, g.dinv, x_out, g.r, g.z, n, PQ,
-                       static_cast<int32_t>(gt), PR[k & 1], static_cast<int32_t>(ge),
+                       static_cast<int32_t>(gt), g.part + kPartOffRz + (k & 1),
                        PR[(k + 1) & 1], st);
             };
             const dim3 blk(kThreads);
@@ -609,8 +626,8 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
     };
     auto update = [&]() {
         hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.p[1],
-                           g.q, g.dinv, g.x, g.r, g.z, n, PQ, static_cast<int32_t>(gt), PR[0],
-                           static_cast<int32_t>(ge), PR[1], g.status);
+                           g.q, g.dinv, g.x, g.r, g.z, n, PQ, static_cast<int32_t>(gt),
+                           g.part + kPartOffRz, PR[1], g.status);
     };
     auto run = [&](int rr) {
         for (int i = 0; i < rr; ++i) {

@@ -210,12 +210,16 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
             }
         }
     }
+    // the stop flag and tolerance as earlier launches left them, loaded ahead
+    // of the reduction's barrier (a load after it would add a round trip)
+    const int32_t done_in = status->done;
+    const double tol_in = status->tol;
     double red_v[3];
     red_v[0] = strided_sum<kThreads, 2>(pr_cur, np_cur, 0.0);
     red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
     red_v[2] = k > 0 ? strided_sum<kThreads, 2>(pr_prev + kPartStride, np_prev, 0.0) : 0.0;
     block_allsum<3, kThreads>(red_v, red);
-    if (status->done) return;
+    if (done_in) return;
     const double rr = red_v[0];
     if (k == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -224,7 +228,7 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
             status->rnorm = sqrt(rr);
         }
     } else if (is_true) {
-        const double tol = status->tol;
+        const double tol = tol_in;
         if (sqrt(rr) < tol) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
                 status->done = 1;
@@ -323,12 +327,13 @@ mixed_update_kernel(const float *__restrict__ p, const float *__restrict__ q,
 #pragma unroll
         for (int c = 0; c < 9; ++c) d[c] = dinv32[9 * static_cast<int64_t>(ii) + c];
     };
+    const int32_t done_in = status->done;  // set by earlier launches only
     if (i0 < n) load(i0);
     double red_v[2];
     red_v[0] = strided_sum<kThreads, 8>(pq_in, np_pq, 0.0);
     red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
     block_allsum<2, kThreads>(red_v, red);
-    if (status->done) return;
+    if (done_in) return;
     if (red_v[0] == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
