@@ -103,6 +103,8 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
     GSLAM_TRY(g.alloc(&g.tile_row, h.tile_row.size()));
     GSLAM_TRY(g.alloc(&g.tile_info, g.n_tiles));
+    GSLAM_TRY(g.alloc(&g.tcol, static_cast<int64_t>(kTileBlocks) * g.n_tiles));
+    GSLAM_TRY(g.alloc(&g.tval, static_cast<int64_t>(9 * kTileBlocks) * g.n_tiles));
     GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
     GSLAM_TRY(g.alloc(&g.bval32, 9 * B));
@@ -156,6 +158,13 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
             ti[t] = make_int4(r0, r1, h.brow_ptr[r0], h.brow_ptr[r1]);
         }
         GSLAM_TRY(g.upload(g.tile_info, ti.data(), static_cast<int64_t>(ti.size()) * 16));
+        std::vector<int32_t> tc(static_cast<size_t>(kTileBlocks) * g.n_tiles);
+        for (int32_t t = 0; t < g.n_tiles; ++t) {
+            const int32_t b0 = h.brow_ptr[h.tile_row[t]], b1 = h.brow_ptr[h.tile_row[t + 1]];
+            for (int32_t l = 0; l < kTileBlocks; ++l)
+                tc[static_cast<size_t>(t) * kTileBlocks + l] = h.bcol[b0 + l < b1 ? b0 + l : b0];
+        }
+        GSLAM_TRY(g.upload(g.tcol, tc.data(), static_cast<int64_t>(tc.size()) * 4));
     }
     GSLAM_TRY(g.upload(g.xoth, h.xoth.data(), 2 * E * 4));
     GSLAM_TRY(g.upload(g.xpos, h.xpos.data(), 2 * E * 4));

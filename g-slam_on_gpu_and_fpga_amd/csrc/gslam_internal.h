@@ -97,6 +97,12 @@ struct DeviceGraph {
     int32_t *tile_row = nullptr;  // SpMV tiles of whole block rows
     int4 *tile_info = nullptr;    // per tile {first row, end row, first block, end block}
     int32_t n_tiles = 0;
+    // The blocks of tile t padded to kTileBlocks slots from slot kTileBlocks t
+    // (column index; a padding slot repeats the tile's first column), so an
+    // SpMV lane finds its block without the tile record: tcol fixed, tval
+    // packed from bval at the start of each FP64 solve (zero in padding).
+    int32_t *tcol = nullptr;
+    double *tval = nullptr;
 
     // assembled system
     double *stage = nullptr;     // kStage per edge

@@ -95,6 +95,23 @@ pcg_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ b
     }
 }
 
+// tval = the blocks of each tile at slots kTileBlocks t + l (zero in the
+// padding slots of a tile with fewer blocks).
+__global__ void __launch_bounds__(kThreads)
+tile_pack_kernel(const int4 *__restrict__ tile_info, const double *__restrict__ bval,
+                 int32_t n_tiles, double *__restrict__ tval)
+{
+    const int64_t n = static_cast<int64_t>(n_tiles) * kTileBlocks * 9;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t slot = e / 9;
+        const int k = static_cast<int>(e - 9 * slot);
+        const int4 ti = tile_info[slot / kTileBlocks];
+        const int64_t b = ti.z + slot % kTileBlocks;
+        tval[e] = b < ti.w ? bval[9 * b + k] : 0.0;
+    }
+}
+
 // A(k): stopping test on the partials of B(k-1), then q = A p with
 // p = z + beta p_old formed per gathered column; p.q partials.
 //
@@ -119,38 +136,41 @@ struct SpmvTileRegs {
 
 __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_info,
                                       const int32_t *__restrict__ brow_ptr,
-                                      const int32_t *__restrict__ bcol,
-                                      const double *__restrict__ bval,
+                                      const int32_t *__restrict__ tcol,
+                                      const double *__restrict__ tval,
                                       const double *__restrict__ z,
                                       const double *__restrict__ p_old, SpmvTileRegs &T)
 {
+    static_assert(kTileBlocks == kThreads, "one lane per padded block slot");
+    // the block of this lane from its padded slot, loaded without waiting for
+    // the tile record (a padding slot holds zeros and a valid column, so its
+    // products are 0); then the tile record and the output row's range and z /
+    // p_old (from a row clamped into the tile: unconditional, so that no
+    // branch waits on them); then the gathered z / p_old and the products
+    const int64_t slot = static_cast<int64_t>(t) * kTileBlocks + threadIdx.x;
+    const int64_t j = 3 * static_cast<int64_t>(tcol[slot]);
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = tval[9 * slot + k];
     const int4 ti = tile_info[t];
     T.r0 = ti.x;
     T.r1 = ti.y;
     T.b0 = ti.z;
     T.b1 = ti.w;
-    const int32_t bb = T.b0 + static_cast<int32_t>(threadIdx.x);
-    T.has_blk = bb < T.b1;
+    T.has_blk = T.b0 + static_cast<int32_t>(threadIdx.x) < T.b1;
     T.has_out = static_cast<int32_t>(threadIdx.x) < 3 * (T.r1 - T.r0);
-    // the output row's range and z / p_old first, from a row clamped into the
-    // tile (unconditional, so that they overlap the block loads below rather
-    // than wait for them)
     const int32_t orow = min(T.r0 + static_cast<int32_t>(threadIdx.x) / 3, T.r1 - 1);
     T.ob0 = brow_ptr[orow];
     T.ob1 = brow_ptr[orow + 1];
     const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
     T.zo = z[o];
     T.po = p_old[o];
-    if (T.has_blk) {
-        const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
-        const double *v = bval + 9 * static_cast<int64_t>(bb);
-        const double z0 = z[j], z1 = z[j + 1], z2 = z[j + 2];
-        const double q0 = p_old[j], q1 = p_old[j + 1], q2 = p_old[j + 2];
+    const double z0 = z[j], z1 = z[j + 1], z2 = z[j + 2];
+    const double q0 = p_old[j], q1 = p_old[j + 1], q2 = p_old[j + 2];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            T.yz[a] = v[3 * a] * z0 + v[3 * a + 1] * z1 + v[3 * a + 2] * z2;
-            T.yp[a] = v[3 * a] * q0 + v[3 * a + 1] * q1 + v[3 * a + 2] * q2;
-        }
+    for (int a = 0; a < 3; ++a) {
+        T.yz[a] = v[3 * a] * z0 + v[3 * a + 1] * z1 + v[3 * a + 2] * z2;
+        T.yp[a] = v[3 * a] * q0 + v[3 * a + 1] * q1 + v[3 * a + 2] * q2;
     }
 }
 
@@ -211,6 +231,7 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
                 const double *__restrict__ bval, const int4 *__restrict__ tile_info,
+                const int32_t *__restrict__ tcol, const double *__restrict__ tval,
                 int32_t n_tiles, const double *__restrict__ z,
                 const double *__restrict__ p_old, double *__restrict__ p_new,
                 double *__restrict__ q, int32_t k, double tol_rel,
@@ -236,7 +257,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
     const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
-    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, bcol, bval, z, p_old, T);
+    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, tcol, tval, z, p_old, T);
     const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     double red_v[2];
     red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
@@ -268,7 +289,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     if (blockIdx.x == 0 && threadIdx.x == 0) rz_hist[k & 1] = rz;
     double pq[1] = {0.0};
     for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
-        if (t != t0) spmv_tile_load(t, tile_info, brow_ptr, bcol, bval, z, p_old, T);
+        if (t != t0) spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, T);
         pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, ys);
     }
     write_partials<1>(pq, pq_out, kSpmvMaxGrid);
@@ -443,6 +464,15 @@ diff_kernel(const double *__restrict__ part, int32_t np, int32_t n, double *__re
     if (threadIdx.x == 0) *out = sqrt(t[0]) / sqrt(3.0 * static_cast<double>(n));
 }
 
+// The padded tile copy of the current block values (before each FP64 solve).
+static void launch_tile_pack(DeviceGraph &g)
+{
+    const int64_t n = static_cast<int64_t>(g.n_tiles) * kTileBlocks * 9;
+    const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(4096, (n + kThreads - 1) / kThreads));
+    hipLaunchKernelGGL(tile_pack_kernel, dim3(grid), dim3(kThreads), 0, g.stream, g.tile_info, g.bval,
+                       g.n_tiles, g.tval);
+}
+
 int launch_spmv64(DeviceGraph &g, const double *x, double *y)
 {
     hipLaunchKernelGGL(spmv64_kernel, dim3(grid_for(g.h.n, kLanesPerRow)), dim3(kThreads), 0,
@@ -498,6 +528,7 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
     double *PQ = g.part + kPartOffPQT;
     PcgStatus *st = g.status;
     GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    launch_tile_pack(g);
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.bdiag, g.bval, b,
                        n, o.max_iter, g.dinv, static_cast<float *>(nullptr), x_out, g.r, g.z,
                        g.p[0], g.p[1], PR[0], st);
@@ -517,7 +548,8 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
             const bool samp = o.profile_every > 0 && k % o.profile_every == 0 && n_samp < kMaxSamples;
             hipEvent_t *ev = samp ? &g.prof_ev[4 * n_samp++] : nullptr;
             auto spmv_args = [&](auto launch) {
-                launch(pcg_spmv_kernel, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.n_tiles, g.z,
+                launch(pcg_spmv_kernel, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval,
+                       g.n_tiles, g.z,
                        pold, pnew, g.q, k, o.tol, PR[k & 1], static_cast<int32_t>(ge),
                        g.part + kPartOffRz, PQ, st);
             };
@@ -620,7 +652,8 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
     if (which < GSLAM_KERNEL_SPMV_FP64 || which > GSLAM_KERNEL_ASSEMBLY) return GSLAM_ERR_ARG;
     auto spmv = [&]() {
         hipLaunchKernelGGL(pcg_spmv_kernel, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
-                           g.bcol, g.bval, g.tile_info, g.n_tiles, g.z, g.p[0], g.p[1], g.q, 0,
+                           g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.n_tiles, g.z, g.p[0],
+                           g.p[1], g.q, 0,
                            1e-30, PR[0], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ,
                            g.status);
     };
@@ -637,6 +670,7 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
         }
     };
     // a consistent solver state: setup, one SpMV (valid p.q partials)
+    launch_tile_pack(g);
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        g.xi, n, 1, g.dinv, static_cast<float *>(nullptr), g.x, g.r, g.z, g.p[0],
                        g.p[1], PR[0], g.status);

@@ -67,6 +67,27 @@ def test_pcg64_converged_against_direct(name):
         assert rel(g.dx(), xs) <= 1e-8
 
 
+def test_pcg64_block_row_longer_than_a_tile():
+    """Pose 0 tied to every other pose by loop closures: its block row holds
+    700 blocks, more than one 256-block SpMV tile, so the FP64 SpMV walks it
+    in three chunks (the first from the padded tile slots, the rest from the
+    BSR arrays).  Converged FP64 PCG against the direct solve."""
+    rng = np.random.default_rng(5)
+    n = 700
+    truth = np.zeros((n, 3))
+    truth[:, 0] = np.arange(n, dtype=float)
+    truth[:, 1] = rng.normal(size=n) * 0.5
+    gc = G._finish(truth, np.zeros(n - 2, dtype=np.int64), np.arange(2, n), rng, 0.05, 0.01)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        assert rp[1] - rp[0] == 3 * n       # scalar row 0 spans all 700 block columns
+        xs = direct(rp, ci, va, xi)
+        st = g.solve("pcg64", 1e-12, 200000)
+        assert st["converged"] and st["true_rel_res"] < 2e-12
+        assert rel(g.dx(), xs) <= 1e-8
+
+
 @pytest.mark.parametrize("name", ["grid800", "park2000"])
 def test_mixed_certifies_true_residual(name):
     gc = {"grid800": lambda: G.grid_city(800, seed=1), "park2000": lambda: G.park(2000, seed=3)}[name]()
