@@ -50,6 +50,28 @@ __device__ inline void block_sum(double (&v)[K], double (*lds)[NT / kWave])
     __syncthreads();
 }
 
+// As block_sum, for the last use of lds in a kernel: no barrier after
+// thread 0 has read the per-wave totals.
+template <int K, int NT>
+__device__ inline void block_sum_last(double (&v)[K], double (*lds)[NT / kWave])
+{
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = wave_sum(v[k]);
+        if (l == 0) lds[k][w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < NT / kWave; ++i) s += lds[k][i];
+            v[k] = s;
+        }
+    }
+}
+
 // Workgroup sum of K values that every thread receives, with one barrier:
 // wave butterflies, per-wave totals to LDS, then each thread adds the totals
 // in wave order (the same bits in every thread and every workgroup that sums

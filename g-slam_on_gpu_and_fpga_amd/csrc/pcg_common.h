@@ -44,11 +44,12 @@ inline unsigned int grid_entries(int64_t n_poses)
 }
 
 // Block sums of K values, written by thread 0 to out[k * stride + blockIdx.x].
+// Called at the end of a kernel (its LDS is not used again).
 template <int K, int NT = kThreads>
 __device__ inline void write_partials(double (&v)[K], double *out, int stride)
 {
     __shared__ double lds[K][NT / kWave];
-    block_sum<K, NT>(v, lds);
+    block_sum_last<K, NT>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) out[k * stride + blockIdx.x] = v[k];
