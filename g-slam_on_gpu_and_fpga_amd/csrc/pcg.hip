@@ -225,7 +225,7 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
     return pq;
 }
 
-// (at most 72 VGPRs: 7 workgroups per CU, 1792 on the chip, keep the 1683
+// (at most 72 VGPRs: 7 workgroups per CU, 1792 on the chip, keep the 1697
 // tiles of the N = 100k grid city in one round; at 92 VGPRs, 5 per CU, the
 // kernel measured 16 us in a solve.  A 64-VGPR cap spills.)
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
