@@ -193,8 +193,10 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
         const int32_t hi = min(T.ob1, T.b0 + kTileBlocks);
         for (int32_t c = max(T.ob0, T.b0); c < hi; ++c) acc += ys[a * kTileBlocks + (c - T.b0)];
     }
-    __syncthreads();
-    // a single block row longer than one tile is walked in further chunks
+    // a single block row longer than one tile is walked in further chunks,
+    // after every thread has summed the first (the condition is the same in
+    // the whole workgroup)
+    if (T.b1 > T.b0 + kTileBlocks) __syncthreads();
     for (int32_t cb = T.b0 + kTileBlocks; cb < T.b1; cb += kTileBlocks) {
         const int32_t bb = cb + static_cast<int32_t>(threadIdx.x);
         if (bb < T.b1) {
@@ -289,7 +291,10 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     if (blockIdx.x == 0 && threadIdx.x == 0) rz_hist[k & 1] = rz;
     double pq[1] = {0.0};
     for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
-        if (t != t0) spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, T);
+        if (t != t0) {
+            __syncthreads();  // the previous tile is summed before ys is rewritten
+            spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, T);
+        }
         pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, ys);
     }
     write_partials<1>(pq, pq_out, kSpmvMaxGrid);
