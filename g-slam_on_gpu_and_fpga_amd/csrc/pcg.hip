@@ -117,17 +117,19 @@ tile_pack_kernel(const int4 *__restrict__ tile_info, const double *__restrict__ 
 //
 // One workgroup per tile of whole block rows (<= 256 blocks, <= 85 rows;
 // tile_info[t] = {first row, end row, first block, end block}): one lane per
-// 3x3 block (its 9 values, column index and the gathered p entries), the
-// block's 3 products staged in LDS, then one thread per output row (3 per
-// block row) sums its row's products in block order.
+// 3x3 block (its 9 values and column index from the tile-padded copy, slot
+// 256 t + lane, and the gathered p entries), the block's 3 products staged in
+// LDS, then one thread per output row (3 per block row) sums its row's
+// products in block order.
 //
 // At N = 100k the matrix and vectors sit in the MI355X Infinity Cache and the
-// kernel is bound by its chain of dependent loads, not by bandwidth: every
-// load that does not need beta (tile record -> block values, column index ->
-// gathered z / p_old, the output row's range and z / p_old) is issued before
-// the partial sums that give beta are loaded and reduced (one barrier).  To
-// hold fewer registers meanwhile, each lane keeps V z_j and V p_old_j (6
-// values instead of 15) and forms V p_j = V z_j + beta V p_old_j afterwards.
+// kernel is bound by its chain of dependent loads, not by bandwidth.  Issue
+// order: the partials that give beta, the lane's slot (column index and
+// values), the tile record and the output row's range and z / p_old, then
+// the gathered z / p_old -- two dependent loads deep; the partials are
+// reduced (one barrier) once the products are formed.  To hold fewer
+// registers meanwhile, each lane keeps V z_j and V p_old_j (6 values instead
+// of 15) and forms V p_j = V z_j + beta V p_old_j afterwards.
 struct SpmvTileRegs {
     int32_t r0, r1, b0, b1, ob0, ob1;
     bool has_blk, has_out;
