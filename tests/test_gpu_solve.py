@@ -88,6 +88,23 @@ def test_pcg64_block_row_longer_than_a_tile():
         assert rel(g.dx(), xs) <= 1e-8
 
 
+def test_pcg64_several_tiles_per_workgroup(oracle):
+    """Grid city N = 300,000: more SpMV tiles (about 5,100) than the 4,096
+    workgroups of the launch, so some workgroups take a second tile.  Fifty
+    FP64 PCG iterations against fifty of the CPU oracle on the same system
+    (downloaded from the device); after so few iterations the two summation
+    orders differ at the rounding level only."""
+    gc = G.grid_city(300000, seed=7)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        st = g.solve("pcg64", 1e-30, 50)
+        assert st["iters"] == 50 and not st["converged"]
+        x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
+        assert it_o == 50
+        assert rel(g.dx(), x_o) <= 1e-9
+
+
 @pytest.mark.parametrize("name", ["grid800", "park2000"])
 def test_mixed_certifies_true_residual(name):
     gc = {"grid800": lambda: G.grid_city(800, seed=1), "park2000": lambda: G.park(2000, seed=3)}[name]()
