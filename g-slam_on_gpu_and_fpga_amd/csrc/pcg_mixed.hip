@@ -181,9 +181,21 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
                   double *__restrict__ pq_out, PcgStatus *status)
 {
     __shared__ double red[3][kThreads / kWave];
+    static_assert(kMaxGrid <= 2 * kThreads, "two partials per thread and array");
     constexpr int kRowsPerTile = kThreads / G;
     const int lane = threadIdx.x % G;
     const int64_t row_first = static_cast<int64_t>(blockIdx.x) * kRowsPerTile + threadIdx.x / G;
+    // The r.r / r.z partials of this and the previous parity first
+    // (np_cur, np_prev <= 2 x kThreads: two per thread and array, summed below
+    // in strided_sum's order), as unconditional loads from clamped indices and
+    // pinned here, so that their round trips overlap the row's loads.
+    const int tid = static_cast<int>(threadIdx.x);
+    const int ia = min(tid, np_cur - 1), ib = min(tid + kThreads, np_cur - 1);
+    const int ja = min(tid, np_prev - 1), jb = min(tid + kThreads, np_prev - 1);
+    const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
+    const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
+    const double zp_a = pr_prev[kPartStride + ja], zp_b = pr_prev[kPartStride + jb];
+    asm volatile("" ::: "memory");
     // loads of the first row handled by this thread (independent of beta)
     float zi[3] = {0.f, 0.f, 0.f}, qi[3] = {0.f, 0.f, 0.f};
     float zo[3] = {0.f, 0.f, 0.f}, qo[3] = {0.f, 0.f, 0.f};
@@ -214,10 +226,12 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
     // of the reduction's barrier (a load after it would add a round trip)
     const int32_t done_in = status->done;
     const double tol_in = status->tol;
+    const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
+    const bool jn_a = tid < np_prev, jn_b = tid + kThreads < np_prev;
     double red_v[3];
-    red_v[0] = strided_sum<kThreads, 2>(pr_cur, np_cur, 0.0);
-    red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
-    red_v[2] = k > 0 ? strided_sum<kThreads, 2>(pr_prev + kPartStride, np_prev, 0.0) : 0.0;
+    red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
+    red_v[1] = (0.0 + (in_a ? rz_a : 0.0)) + (in_b ? rz_b : 0.0);
+    red_v[2] = k > 0 ? (0.0 + (jn_a ? zp_a : 0.0)) + (jn_b ? zp_b : 0.0) : 0.0;
     block_allsum<3, kThreads>(red_v, red);
     if (done_in) return;
     const double rr = red_v[0];
