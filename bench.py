@@ -25,7 +25,8 @@ Also reported:
   same GN step, extrapolated to the FP64 solve's iteration count;
 * gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) and in
   FP64 (GJ_BLOCKED64) against the plain FP64 Gauss-Jordan and
-  numpy.linalg.solve (FP64), grid-city N=1600;
+  numpy.linalg.solve (FP64), grid-city N=800, 1600 and 5000 (3N = 2400,
+  4800, 15000; the plain FP64 Gauss-Jordan up to N=2000);
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -353,7 +354,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=1500,
                     help="PCG iterations of the bounded CPU oracle sample (0: skip)")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--gj-n", default="1600", help="grid-city sizes of the GJ section ('' skips)")
+    ap.add_argument("--gj-n", default="800,1600,5000",
+                    help="grid-city sizes of the GJ section ('' skips; --profile-kernel "
+                         "gj_trailing uses the first)")
     ap.add_argument("--gj64-max-n", type=int, default=2000)
     ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
