@@ -351,7 +351,7 @@ def main():
     ap.add_argument("--solvers", default="mixed,pcg64")
     ap.add_argument("--profile-every", type=int, default=97,
                     help="sample every k-th SpMV/update launch of the timed FP64 solves")
-    ap.add_argument("--cpu-iters", type=int, default=1500,
+    ap.add_argument("--cpu-iters", type=int, default=5000,
                     help="PCG iterations of the bounded CPU oracle sample (0: skip)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--gj-n", default="800,1600,5000",
