@@ -101,13 +101,11 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.csr_col_idx, h.nnz));
     GSLAM_TRY(g.alloc(&g.csr_pos, 9 * B));
     g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
-    GSLAM_TRY(g.alloc(&g.tile_row, h.tile_row.size()));
     GSLAM_TRY(g.alloc(&g.tile_info, g.n_tiles));
     GSLAM_TRY(g.alloc(&g.tcol, static_cast<int64_t>(kTileBlocks) * g.n_tiles));
     GSLAM_TRY(g.alloc(&g.tval, static_cast<int64_t>(9 * kTileBlocks) * g.n_tiles));
     GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
-    GSLAM_TRY(g.alloc(&g.bval32, 9 * B));
     GSLAM_TRY(g.alloc(&g.xi, dim));
     GSLAM_TRY(g.alloc(&g.x, dim));
     GSLAM_TRY(g.alloc(&g.r, dim));
@@ -122,7 +120,6 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.q32, dim));
     GSLAM_TRY(g.alloc(&g.p32[0], dim));
     GSLAM_TRY(g.alloc(&g.p32[1], dim));
-    GSLAM_TRY(g.alloc(&g.r64, dim));
     GSLAM_TRY(g.alloc(&g.xoth, 2 * E));
     GSLAM_TRY(g.alloc(&g.xpos, 2 * E));
     GSLAM_TRY(g.alloc(&g.frec, 16 * E));
@@ -150,7 +147,6 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.upload(g.csr_row_ptr, h.csr_row_ptr.data(), (dim + 1) * 4));
     GSLAM_TRY(g.upload(g.csr_col_idx, h.csr_col_idx.data(), h.nnz * 4));
     GSLAM_TRY(g.upload(g.csr_pos, h.csr_pos.data(), 9 * B * 4));
-    GSLAM_TRY(g.upload(g.tile_row, h.tile_row.data(), h.tile_row.size() * 4));
     {
         std::vector<int4> ti(static_cast<size_t>(g.n_tiles));
         for (int32_t t = 0; t < g.n_tiles; ++t) {

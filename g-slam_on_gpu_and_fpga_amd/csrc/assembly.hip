@@ -260,7 +260,7 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
 __global__ void __launch_bounds__(kThreads)
 gather_blocks_kernel(const int32_t *__restrict__ cptr, const int32_t *__restrict__ clist,
                      const double *__restrict__ stage, int64_t n_vals, int64_t diag0,
-                     double anchor, double *__restrict__ bval, float *__restrict__ bval32)
+                     double anchor, double *__restrict__ bval)
 {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= n_vals) return;
@@ -273,7 +273,6 @@ gather_blocks_kernel(const int32_t *__restrict__ cptr, const int32_t *__restrict
         v += stage[static_cast<int64_t>(ent >> 2) * kStage + (ent & 3) * 9 + k];
     }
     bval[t] = v;
-    if (bval32) bval32[t] = static_cast<float>(v);
 }
 
 // xi entries: contributions in edge order; with the anchor prior, pose 0
@@ -310,14 +309,12 @@ bsr_to_csr_kernel(const int32_t *__restrict__ pos, const double *__restrict__ bv
 
 __global__ void __launch_bounds__(kThreads)
 csr_to_bsr_kernel(const int32_t *__restrict__ pos, const double *__restrict__ csr,
-                  int64_t n_vals, double *__restrict__ bval, float *__restrict__ bval32)
+                  int64_t n_vals, double *__restrict__ bval)
 {
     const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= n_vals) return;
     const int32_t p = pos[t];
-    const double v = p >= 0 ? csr[p] : 0.0;
-    bval[t] = v;
-    if (bval32) bval32[t] = static_cast<float>(v);
+    bval[t] = p >= 0 ? csr[p] : 0.0;
 }
 
 static inline unsigned int nblk(int64_t n) { return static_cast<unsigned int>((n + kThreads - 1) / kThreads); }
@@ -344,7 +341,7 @@ int launch_assembly(DeviceGraph &g, bool exact_hessian)
     const int64_t nv = 9 * h.n_blocks;
     hipLaunchKernelGGL(gather_blocks_kernel, dim3(nblk(nv)), dim3(kThreads), 0, g.stream,
                        g.cptr, g.clist, g.stage, nv, static_cast<int64_t>(h.bdiag[0]), g.anchor,
-                       g.bval, g.bval32);
+                       g.bval);
     const int64_t dim = 3 * static_cast<int64_t>(h.n);
     hipLaunchKernelGGL(gather_xi_kernel, dim3(nblk(dim)), dim3(kThreads), 0, g.stream, g.xptr,
                        g.xlist, g.stage, dim, g.xi, g.poses, g.anchor_prior, g.anchor,
@@ -366,7 +363,7 @@ int launch_csr_to_bsr(DeviceGraph &g, const double *csr_vals, double *bval)
 {
     const int64_t nv = 9 * g.h.n_blocks;
     hipLaunchKernelGGL(csr_to_bsr_kernel, dim3(nblk(nv)), dim3(kThreads), 0, g.stream,
-                       g.csr_pos, csr_vals, nv, bval, g.bval32);
+                       g.csr_pos, csr_vals, nv, bval);
     GSLAM_HIP(hipGetLastError());
     return GSLAM_OK;
 }

@@ -94,7 +94,6 @@ struct DeviceGraph {
     int32_t *brow_ptr = nullptr, *bcol = nullptr, *bdiag = nullptr;
     int32_t *cptr = nullptr, *clist = nullptr, *xptr = nullptr, *xlist = nullptr;
     int32_t *csr_row_ptr = nullptr, *csr_col_idx = nullptr, *csr_pos = nullptr;
-    int32_t *tile_row = nullptr;  // SpMV tiles of whole block rows
     int4 *tile_info = nullptr;    // per tile {first row, end row, first block, end block}
     int32_t n_tiles = 0;
     // The blocks of tile t padded to kTileBlocks slots from slot kTileBlocks t
@@ -107,7 +106,6 @@ struct DeviceGraph {
     // assembled system
     double *stage = nullptr;     // kStage per edge
     double *bval = nullptr;      // 9 per block, FP64
-    float *bval32 = nullptr;     // 9 per block, FP32 copy (mixed solver)
     double *xi = nullptr;        // 3N
 
     // solver work space
@@ -116,7 +114,6 @@ struct DeviceGraph {
     double *dinv = nullptr;      // FP64 D^-1, column-major per entry: [c][3i+a]
     float *dinv32 = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *p32[2] = {nullptr, nullptr};
-    double *r64 = nullptr;       // mixed: FP64 true residual scratch
     double *part = nullptr;      // reduction partials (pcg_common.h layout)
     PcgStatus *status = nullptr;
     PcgStatus *status_host = nullptr;  // pinned mirror
