@@ -61,6 +61,13 @@ def spmv_bytes(n_poses, n_blocks):
     return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 4 * 24 * n_poses
 
 
+def update_bytes(n_poses):
+    """Algorithmic bytes of one FP64 PCG update launch: x, r, p, q and the
+    3x3 block of D^-1 read (4 x 24 + 72 B per pose), x, r and z written (3 x 24
+    B per pose)."""
+    return n_poses * (4 * 24 + 72 + 3 * 24)
+
+
 def measured_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -319,14 +326,16 @@ def profile_kernel(args, gslam, G):
         log(f"stream: 2 GiB copy {us:.0f} us, {gbs:.0f} GB/s read+write = "
             f"{100 * gbs / HBM_PEAK_GBS:.1f}% of the 8 TB/s HBM peak")
         return
-    if args.profile_kernel == "spmv64":
+    if args.profile_kernel in ("spmv64", "update64"):
+        k = args.profile_kernel
         gc = G.grid_city(args.n, seed=args.seed)
         with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
             g.assemble()
-            us = g.time_kernel("spmv64", 20)
-            byts = spmv_bytes(g.info["n_poses"], g.info["n_blocks"])
+            us = g.time_kernel(k, 20)
+            byts = (spmv_bytes(g.info["n_poses"], g.info["n_blocks"]) if k == "spmv64"
+                    else update_bytes(g.info["n_poses"]))
             gbs = byts / (us * 1e-6) / 1e9
-            log(f"spmv64 N={args.n}: {us:.2f} us/launch (23 launches), {byts / 1e6:.1f} MB "
+            log(f"{k} N={args.n}: {us:.2f} us/launch (23 launches), {byts / 1e6:.1f} MB "
                 f"algorithmic, {gbs:.0f} GB/s = {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
         return
     N = int(args.gj_n.split(",")[0] or 1600)
@@ -361,7 +370,7 @@ def main():
     ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
     ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
-    ap.add_argument("--profile-kernel", choices=["spmv64", "gj_trailing", "copy", "stream"],
+    ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "gj_trailing", "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "
                          "passes; 'copy' is a 64 MiB device copy for calibrating the counters, "
                          "'stream' times a 2 GiB device copy: the achievable HBM bandwidth)")
