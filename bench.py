@@ -22,7 +22,9 @@ Also reported:
   algorithmic bytes and the HBM fraction; HBM traffic per launch from the
   committed rocprofv3 counter summary when present;
 * cpu_baseline: the CPU oracle (FP64, one thread) on a bounded sample of the
-  same GN step, extrapolated to the FP64 solve's iteration count;
+  same GN step, extrapolated to the FP64 solve's iteration count; and, as
+  whole processes on the reference's full victoria-park and cityTrees10k
+  logs, the reference's own CPU executable beside the drop-in GSLAM;
 * gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) and in
   FP64 (GJ_BLOCKED64) against the plain FP64 Gauss-Jordan and
   numpy.linalg.solve (FP64), grid-city N=800, 1600 and 5000 (3N = 2400,
@@ -33,9 +35,12 @@ Also reported:
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
+import gzip
 import json
 import os
 import re
+import shutil
+import subprocess
 import sys
 import time
 
@@ -176,11 +181,56 @@ def cpu_section(args, gc, init, info, results):
     cpu_ms = 1000.0 * (times[0] + t_iter * full_iters)
     log(f"cpu oracle: {cpu_ms:.0f} ms per GN step (extrapolated)")
     return dict(value=cpu_ms, unit="ms per GN step", cores=1, kind="port",
+                reference_cli=reference_cli(args),
                 sample=(f"CPU oracle (oracle/gslam_oracle.c, FP64, 1 thread) GN step on the same "
                         f"N={info['n_poses']} graph: edges + CSR assembly + pose update timed in "
                         f"full ({1000 * times[0]:.0f} ms), block-Jacobi PCG timed for "
                         f"{args.cpu_iters} iterations ({1000 * t_iter:.2f} ms/iteration) and "
                         f"extrapolated to the GPU FP64 solve's {full_iters} iterations"))
+
+
+def _dataset(name):
+    """Path of a reference data set shipped (gzip) under tests/golden/data,
+    inflated once into _build/data."""
+    src = os.path.join(ROOT, "tests", "golden", "data", name)
+    if os.path.exists(src):
+        return src
+    out = os.path.join(ROOT, "_build", "data", name)
+    if not os.path.exists(out):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        with gzip.open(src + ".gz", "rb") as fi, open(out + ".tmp", "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+        os.replace(out + ".tmp", out)
+    return out
+
+
+def reference_cli(args):
+    """Whole processes on the reference's own full data sets: the reference's
+    CPU executable (oracle/_ref/GSLAM_cpu_pcg_fp64, its cpu_pcg_fp64 sources
+    compiled by oracle/Makefile, one thread) against the drop-in GSLAM on the
+    GPU, `<exe> 1 0.1 <rounds> 0 <file>` each (file reading, edges, PCG at the
+    reference's 1e-6 / 500 iterations, pose updates), with the per-round diff
+    lines both print."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "GSLAM_cpu_pcg_fp64")
+    ours = os.path.join(ROOT, "g-slam_on_gpu_and_fpga_amd", "bin", "GSLAM")
+    if args.ref_cli_rounds <= 0 or not (os.path.exists(ref) and os.path.exists(ours)):
+        return None
+    out = dict(rounds=args.ref_cli_rounds, cores=1)
+    for name in ("victoria-park.log.txt", "cityTrees10k.log.txt"):
+        path = _dataset(name)
+        row = {}
+        for tag, exe in (("reference_cpu", ref), ("gslam_gpu", ours)):
+            t = time.perf_counter()
+            r = subprocess.run([exe, "1", "0.1", str(args.ref_cli_rounds), "0", path],
+                               capture_output=True, text=True, timeout=900)
+            row[f"{tag}_s"] = time.perf_counter() - t
+            row[f"{tag}_diffs"] = [float(l.split(":")[1]) for l in r.stdout.splitlines()
+                                   if "rounds executed" in l]
+        out[name] = row
+        log(f"{name}, {args.ref_cli_rounds} GN rounds: reference CPU executable "
+            f"{row['reference_cpu_s']:.2f} s, GSLAM on the GPU {row['gslam_gpu_s']:.2f} s "
+            f"(diffs {row['reference_cpu_diffs']} vs {row['gslam_gpu_diffs']})")
+    return out
 
 
 def gj_section(args, gslam, G, np):
@@ -362,6 +412,8 @@ def main():
                     help="sample every k-th SpMV/update launch of the timed FP64 solves")
     ap.add_argument("--cpu-iters", type=int, default=5000,
                     help="PCG iterations of the bounded CPU oracle sample (0: skip)")
+    ap.add_argument("--ref-cli-rounds", type=int, default=3,
+                    help="GN rounds of the reference executable vs GSLAM comparison (0: skip)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--gj-n", default="800,1600,5000",
                     help="grid-city sizes of the GJ section ('' skips; --profile-kernel "
