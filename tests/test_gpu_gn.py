@@ -121,6 +121,22 @@ def test_cli_drop_in(tmp_path):
     assert bad.returncode == 1 and bad.stdout.startswith("Usage: ")
 
 
+def test_cli_use_diff_is_ignored_as_by_the_gpu_reference(tmp_path):
+    """The reference's GPU PCG driver reads use_diff but always stops on the
+    diff rule (src/gpu_pcg_fp64/GSLAM.cu:165-171); GSLAM does the same, so
+    use_diff 0 and 1 print the same rounds and dump the same poses."""
+    path = data_path("cityTrees100.log.txt")
+    runs = {}
+    for ud in ("0", "1"):
+        dump = tmp_path / f"poses_{ud}.txt"
+        out = subprocess.run([GSLAM_BIN, ud, "0.1", "3", "1", path], capture_output=True,
+                             text=True, timeout=120, check=True,
+                             env=dict(os.environ, POSE_DUMP_FILE=str(dump)))
+        runs[ud] = (out.stdout.splitlines()[1:], np.loadtxt(dump))
+    assert runs["0"][0] == runs["1"][0]
+    assert np.array_equal(runs["0"][1], runs["1"][1])
+
+
 def test_cli_dense_solvers_agree(tmp_path):
     """GSLAM_SOLVER=gj64b (blocked FP64) against gj64 (one pivot per launch):
     the same unpivoted elimination, so the same poses after 3 rounds to the
