@@ -50,6 +50,20 @@ def test_inverse_against_numpy_and_gj64(n):
             assert rel(inv, gslam.gj_invert(a, "gj64")) <= 1e-12
 
 
+def test_inverse_tail_tiles_of_the_pivot_workgroup():
+    """n = 9200: 72 panels, 5111 tiles per trailing launch besides the next
+    pivot block, 510 other tile workers; the pivot workgroup's two workers
+    take the last 11 tiles after the pivot block.  a = I + 0.25 G / sqrt(n)
+    (singular values in [0.5, 1.5]); one trailing tile update of one block
+    step left out gives 9.6e-5 against numpy.linalg.inv (numpy emulation)."""
+    n = 9200
+    rng = np.random.default_rng(n)
+    a = np.eye(n) + 0.25 * rng.normal(size=(n, n)) / np.sqrt(n)
+    inv = gslam.gj_invert(a, "gj64_blocked")
+    assert rel(inv, np.linalg.inv(a)) <= 1e-11, rel(inv, np.linalg.inv(a))
+    assert np.abs(inv @ a - np.eye(n)).max() <= 1e-10
+
+
 def test_diagonal_spread():
     n = 400
     rng = np.random.default_rng(3)

@@ -77,6 +77,21 @@ def test_inverse_many_tiles_per_workgroup(n):
     assert np.abs(inv @ a - np.eye(n)).max() <= 1e-4
 
 
+def test_inverse_tail_tiles_of_the_pivot_workgroup():
+    """n = 9200: 72 panels, 5111 tiles per trailing launch besides the next
+    pivot block, 255 other workgroups; the pivot workgroup takes the last 11
+    tiles after the pivot block.  a = I + 0.25 G / sqrt(n) (singular values
+    in [0.5, 1.5]).  A numpy emulation of this algorithm gives 2.5e-7 against
+    numpy.linalg.inv and 1.0e-4 when one trailing tile update of one block
+    step is left out (three positions tried)."""
+    n = 9200
+    rng = np.random.default_rng(n)
+    a = np.eye(n) + 0.25 * rng.normal(size=(n, n)) / np.sqrt(n)
+    inv = gslam.gj_invert(a, "gj_mixed")
+    assert rel(inv, np.linalg.inv(a)) <= 5e-6, rel(inv, np.linalg.inv(a))
+    assert np.abs(inv @ a - np.eye(n)).max() <= 2e-5
+
+
 def test_zero_pivot_gives_zero_matrix():
     z = np.eye(6)
     z[3, 3] = 0.0
