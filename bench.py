@@ -22,7 +22,8 @@ Also reported:
   algorithmic bytes and the HBM fraction; HBM traffic per launch from the
   committed rocprofv3 counter summary when present;
 * cpu_baseline: the CPU oracle (FP64, one thread) on a bounded sample of the
-  same GN step, extrapolated to the FP64 solve's iteration count; and, as
+  same GN step, extrapolated to the FP64 solve's iteration count; the same
+  with the PCG loops on all OpenMP threads (omp); and, as
   whole processes on the reference's full victoria-park and cityTrees10k
   logs, the reference's own CPU executable beside the drop-in GSLAM;
 * gj: blocked Gauss-Jordan with FP32 MFMA trailing updates (GJ_MIXED) and in
@@ -162,13 +163,10 @@ def pcg_section(args, gslam, G, np):
     return gc, init, info, results, roofline, kernels
 
 
-def cpu_section(args, gc, init, info, results):
-    if args.cpu_iters <= 0 or "pcg64" not in results:
-        return None
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
-    O = Oracle()
-    rp, ci = O.csr_pattern(info["n_poses"], gc["i"], gc["j"])
+def _cpu_gn_ms(O, args, gc, init, rp, ci, full_iters):
+    """One CPU oracle GN step: edges + CSR assembly + pose update timed in full
+    (0 PCG iterations), PCG timed for args.cpu_iters iterations and
+    extrapolated to full_iters."""
     times = {}
     for it in (0, args.cpu_iters):
         p = init.copy()
@@ -177,14 +175,34 @@ def cpu_section(args, gc, init, info, results):
                           tol=1e-300, max_iter=it, wrap=1)
         times[it] = time.perf_counter() - t
     t_iter = (times[args.cpu_iters] - times[0]) / args.cpu_iters
+    return 1000.0 * (times[0] + t_iter * full_iters), times[0], t_iter
+
+
+def cpu_section(args, gc, init, info, results):
+    if args.cpu_iters <= 0 or "pcg64" not in results:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import LIB_OMP, Oracle
+    O = Oracle()
+    rp, ci = O.csr_pattern(info["n_poses"], gc["i"], gc["j"])
     full_iters = results["pcg64"]["pcg_iters"]
-    cpu_ms = 1000.0 * (times[0] + t_iter * full_iters)
+    cpu_ms, t0, t_iter = _cpu_gn_ms(O, args, gc, init, rp, ci, full_iters)
     log(f"cpu oracle: {cpu_ms:.0f} ms per GN step (extrapolated)")
+    # the same step with the PCG loops on all host threads (liboracle_omp.so)
+    P = Oracle(LIB_OMP)
+    omp_ms, o0, o_iter = _cpu_gn_ms(P, args, gc, init, rp, ci, full_iters)
+    log(f"cpu oracle, OpenMP {P.threads()} threads: {omp_ms:.0f} ms per GN step "
+        f"({1000 * o_iter:.3f} ms/iteration)")
+    omp = dict(value=omp_ms, unit="ms per GN step", cores=P.threads(), kind="port",
+               sample=(f"oracle/liboracle_omp.so: the same step with the PCG loops on "
+                       f"{P.threads()} OpenMP threads (edges, assembly and pose update on one), "
+                       f"{1000 * o0:.0f} ms fixed + {args.cpu_iters} iterations at "
+                       f"{1000 * o_iter:.3f} ms/iteration, extrapolated to {full_iters}"))
     return dict(value=cpu_ms, unit="ms per GN step", cores=1, kind="port",
-                reference_cli=reference_cli(args),
+                reference_cli=reference_cli(args), omp=omp,
                 sample=(f"CPU oracle (oracle/gslam_oracle.c, FP64, 1 thread) GN step on the same "
                         f"N={info['n_poses']} graph: edges + CSR assembly + pose update timed in "
-                        f"full ({1000 * times[0]:.0f} ms), block-Jacobi PCG timed for "
+                        f"full ({1000 * t0:.0f} ms), block-Jacobi PCG timed for "
                         f"{args.cpu_iters} iterations ({1000 * t_iter:.2f} ms/iteration) and "
                         f"extrapolated to the GPU FP64 solve's {full_iters} iterations"))
 

@@ -20,6 +20,11 @@
  *   orc_between_edge / orc_between_edges / orc_between_gn_step: no reference
  *   counterpart; pinned by central finite differences of the residual
  *   (J), J^T L J structure, and a direct sparse solve (tests).
+ *
+ * The OpenMP pragmas in the PCG loops take effect only in liboracle_omp.so
+ * (the same file built with -fopenmp), a multi-core CPU timing baseline whose
+ * dot products are summed in another order: within rounding of liboracle.so
+ * (tests/test_oracle_omp.py), not bitwise.  liboracle.so ignores them.
  */
 #include "gslam_oracle.h"
 
@@ -27,6 +32,20 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads the PCG loops use: 1 in liboracle.so, the OpenMP team size in
+ * liboracle_omp.so. */
+int32_t orc_threads(void)
+{
+#ifdef _OPENMP
+    return (int32_t)omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------------------------------------ */
 /* Generic small dense helpers with the reference's loop/summation order.
@@ -643,6 +662,7 @@ void orc_block_jacobi(int32_t dim, const int32_t *row_ptr,
 void orc_spmv(int32_t dim, const int32_t *row_ptr, const int32_t *col_idx,
               const double *values, const double *x, double *y)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < dim; i++) {
         y[i] = 0.0;
         for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++)
@@ -654,6 +674,7 @@ static void apply_bj(int64_t dim, const double *diag_inv, const double *r,
                      double *z)
 {
     int64_t nb = dim / 3;
+#pragma omp parallel for schedule(static)
     for (int64_t blk = 0; blk < nb; blk++) {
         int64_t base = blk * 3;
         const double *inv = diag_inv + blk * 9;
@@ -689,10 +710,12 @@ int32_t orc_pcg(int32_t dim, const int32_t *row_ptr, const int32_t *col_idx,
     for (int32_t k = 0; k < max_iter; k++) {                        /* :347 */
         orc_spmv(dim, row_ptr, col_idx, values, p, q);
         double pq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : pq)
         for (int64_t i = 0; i < n; i++) pq += p[i] * q[i];
         if (pq == 0.0) break;                                       /* :357 */
         double alpha = rz_old / pq;
         double r_norm = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : r_norm)
         for (int64_t i = 0; i < n; i++) {
             x[i] += alpha * p[i];
             r[i] -= alpha * q[i];
@@ -703,8 +726,10 @@ int32_t orc_pcg(int32_t dim, const int32_t *row_ptr, const int32_t *col_idx,
         if (sqrt(r_norm) < tol) break;                              /* :366 */
         apply_bj(n, diag_inv, r, z);
         double rz_new = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rz_new)
         for (int64_t i = 0; i < n; i++) rz_new += r[i] * z[i];
         double beta = rz_new / rz_old;
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
         rz_old = rz_new;
     }

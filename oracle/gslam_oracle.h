@@ -95,6 +95,9 @@ int orc_dense_values(int32_t n_poses, int64_t n_edges, const int32_t *t1,
 /* tol_rel and max_iter replace the hard-coded 1e-6 and 500 (MatrixOps.c:345,
  * 347).  res_hist (max_iter, may be NULL) receives ||r|| after each update.
  * Returns the number of iterations executed (the loop index reached). */
+/* Threads the PCG loops run on: 1 in liboracle.so, the OpenMP team size in
+ * liboracle_omp.so (the same source built with -fopenmp). */
+int32_t orc_threads(void);
 int32_t orc_pcg(int32_t dim, const int32_t *row_ptr, const int32_t *col_idx,
                 const double *values, const double *b, double tol_rel,
                 int32_t max_iter, double *x, double *res_hist);

@@ -18,6 +18,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")
 REF_DIR = os.path.join(HERE, "_ref")
 EDGE_VALS = 42
 
@@ -35,8 +36,9 @@ def _ptr(a):
 
 def build(force: bool = False) -> None:
     """Compile liboracle.so (and _ref when the reference sources exist)."""
-    if force or not os.path.exists(LIB):
-        subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if force or not (os.path.exists(LIB) and os.path.exists(LIB_OMP)):
+        subprocess.run(["make", "-s", "-C", HERE, "liboracle.so", "liboracle_omp.so"],
+                       check=True)
     ref = os.environ.get("GSLAM_REFERENCE", "/root/reference")
     if os.path.isdir(os.path.join(ref, "src", "cpu_pcg_fp64")):
         subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={ref}"], check=True)
@@ -70,6 +72,8 @@ class Oracle:
                                        vp, vp]
         L.orc_pcg.argtypes = [C.c_int32, vp, vp, vp, vp, C.c_double, C.c_int32, vp, vp]
         L.orc_pcg.restype = C.c_int32
+        L.orc_threads.argtypes = []
+        L.orc_threads.restype = C.c_int32
         L.orc_block_jacobi.argtypes = [C.c_int32, vp, vp, vp, vp]
         L.orc_spmv.argtypes = [C.c_int32, vp, vp, vp, vp, vp]
         L.orc_gj_inv.argtypes = [C.c_int32, vp, vp]
@@ -215,6 +219,10 @@ class Oracle:
         return om, xi
 
     # ---------------------------------------------------------------- solve
+    def threads(self) -> int:
+        """Threads of the PCG loops (1 unless this is liboracle_omp.so)."""
+        return int(self.L.orc_threads())
+
     def pcg(self, rp, ci, vals, b, tol=1e-6, max_iter=500, hist=False):
         dim = len(rp) - 1
         x = np.zeros(dim)
