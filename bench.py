@@ -366,6 +366,26 @@ def adjoint_section(args, gslam, G, np):
     return out
 
 
+def rcm_relabel(gc):
+    """The grid-city graph with its poses relabelled in reverse Cuthill-McKee
+    order of the edge graph (scipy), so that edges join nearby labels."""
+    import numpy as np
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    n = len(gc["poses"]) // 3
+    i, j = gc["i"], gc["j"]
+    a = sp.coo_matrix((np.ones(len(i)), (i, j)), shape=(n, n)).tocsr()
+    perm = reverse_cuthill_mckee(a + a.T, symmetric_mode=True)   # new -> old
+    new = np.empty(n, np.int64)
+    new[perm] = np.arange(n)
+    out = dict(gc)
+    out["poses"] = gc["poses"].reshape(-1, 3)[perm].ravel().copy()
+    out["i"], out["j"] = new[i].astype(i.dtype), new[j].astype(j.dtype)
+    log(f"rcm: mean |i - j| over the edges {np.abs(i - j).mean():.0f} -> "
+        f"{np.abs(out['i'] - out['j']).mean():.0f}")
+    return out
+
+
 def profile_kernel(args, gslam, G):
     """A few launches of one kernel for counter collection."""
     if args.profile_kernel == "copy":
@@ -397,6 +417,8 @@ def profile_kernel(args, gslam, G):
     if args.profile_kernel in ("spmv64", "update64"):
         k = args.profile_kernel
         gc = G.grid_city(args.n, seed=args.seed)
+        if args.order == "rcm":
+            gc = rcm_relabel(gc)
         with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
             g.assemble()
             us = g.time_kernel(k, 20)
@@ -421,6 +443,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--order", choices=["gen", "rcm"], default="gen",
+                    help="pose labels for --profile-kernel spmv64/update64: generator order "
+                         "or reverse Cuthill-McKee")
     ap.add_argument("--tol", type=float, default=3e-13, help="FP64 PCG tolerance")
     ap.add_argument("--mixed-tol", type=float, default=1e-12)
     ap.add_argument("--ref-tol", type=float, default=1e-14)
