@@ -99,7 +99,8 @@ struct DeviceGraph {
     // The blocks of tile t padded to kTileBlocks slots from slot kTileBlocks t
     // (column index; a padding slot repeats the tile's first column), so an
     // SpMV lane finds its block without the tile record: tcol fixed, tval
-    // packed from bval at the start of each FP64 solve (zero in padding).
+    // packed from bval at the start of each FP64 solve (zero in padding),
+    // entry k of slot l of tile t at (9 t + k) kTileBlocks + l.
     int32_t *tcol = nullptr;
     double *tval = nullptr;
 

@@ -95,8 +95,10 @@ pcg_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ b
     }
 }
 
-// tval = the blocks of each tile at slots kTileBlocks t + l (zero in the
-// padding slots of a tile with fewer blocks).
+// tval = the blocks of each tile, entry k of slot l at (9 t + k) kTileBlocks
+// + l (plane-major within the tile, so that each of an SpMV lane's nine loads
+// is one contiguous 512-byte run per wave; zero in the padding slots of a
+// tile with fewer blocks).
 __global__ void __launch_bounds__(kThreads)
 tile_pack_kernel(const int4 *__restrict__ tile_info, const double *__restrict__ bval,
                  int32_t n_tiles, double *__restrict__ tval)
@@ -104,10 +106,11 @@ tile_pack_kernel(const int4 *__restrict__ tile_info, const double *__restrict__ 
     const int64_t n = static_cast<int64_t>(n_tiles) * kTileBlocks * 9;
     for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n;
          e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int64_t slot = e / 9;
-        const int k = static_cast<int>(e - 9 * slot);
-        const int4 ti = tile_info[slot / kTileBlocks];
-        const int64_t b = ti.z + slot % kTileBlocks;
+        const int64_t plane = e / kTileBlocks;  // 9 t + k
+        const int64_t t = plane / 9;
+        const int k = static_cast<int>(plane - 9 * t);
+        const int4 ti = tile_info[t];
+        const int64_t b = ti.z + (e - plane * kTileBlocks);
         tval[e] = b < ti.w ? bval[9 * b + k] : 0.0;
     }
 }
@@ -151,9 +154,10 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     // branch waits on them); then the gathered z / p_old and the products
     const int64_t slot = static_cast<int64_t>(t) * kTileBlocks + threadIdx.x;
     const int64_t j = 3 * static_cast<int64_t>(tcol[slot]);
+    const double *tv = tval + 9 * static_cast<int64_t>(t) * kTileBlocks + threadIdx.x;
     double v[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = tval[9 * slot + k];
+    for (int k = 0; k < 9; ++k) v[k] = tv[k * kTileBlocks];
     const int4 ti = tile_info[t];
     T.r0 = ti.x;
     T.r1 = ti.y;
