@@ -117,7 +117,10 @@ def pcg_section(args, gslam, G, np):
             st = g.solve(solver, tol, args.max_iter)
             dx = g.dx()
             err = float(np.linalg.norm(dx - dx_ref) / np.linalg.norm(dx_ref))
-            acc = dict(rel_dx_err=err, meets_1e8=err <= 1e-8, max_abs_dx_err=float(np.abs(dx - dx_ref).max()),
+            d3 = (dx - dx_ref).reshape(-1, 3)
+            acc = dict(rel_dx_err=err, meets_1e8=err <= 1e-8, max_abs_dx_err=float(np.abs(d3).max()),
+                       max_pose_err_m=float(np.abs(d3[:, :2]).max()),
+                       max_pose_err_rad=float(np.abs(d3[:, 2]).max()),
                        true_rel_res=st["true_rel_res"], converged=st["converged"],
                        refreshes=st["reliable_updates"])
             log(f"{solver}: accuracy {acc}")
