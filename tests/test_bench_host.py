@@ -1,0 +1,36 @@
+"""Host-side pieces of bench.py that need no GPU: the algorithmic byte counts
+the roofline figures use, and the reverse Cuthill-McKee relabelling behind
+`--order rcm`.  The relabelled graph must be the same graph (a permutation of
+the pose labels carrying poses and edge ends together), with nearby labels
+on every edge."""
+import numpy as np
+
+import bench
+from gslam import generators as G
+
+
+def test_byte_counts():
+    # per block 9 FP64 values + an int32 column; per row the int32 pointer,
+    # z and p_old read and p_new and q written (4 x 3 doubles)
+    assert bench.spmv_bytes(1, 1) == 76 + 8 + 96
+    assert bench.spmv_bytes(100000, 430854) == 42744908
+    # per pose x, r, p, q and D^-1 read, x, r, z written
+    assert bench.update_bytes(100000) == 24000000
+
+
+def test_rcm_relabel_is_a_permutation():
+    gc = G.grid_city(3000, seed=2)
+    out = bench.rcm_relabel(gc)
+    n = len(gc["poses"]) // 3
+    p0 = gc["poses"].reshape(-1, 3)
+    p1 = out["poses"].reshape(-1, 3)
+    # every pose appears once under its new label: recover the permutation
+    # from the edges and check poses and edge ends move together
+    new_of = np.full(n, -1)
+    new_of[gc["i"]] = out["i"]
+    new_of[gc["j"]] = out["j"]
+    assert (new_of >= 0).all() and len(np.unique(new_of)) == n
+    assert np.array_equal(p1[new_of], p0)
+    assert np.array_equal(out["meas"], gc["meas"])
+    # nearby labels: the mean label distance over the edges shrinks
+    assert np.abs(out["i"] - out["j"]).mean() < 0.2 * np.abs(gc["i"] - gc["j"]).mean()
