@@ -43,17 +43,33 @@ inline unsigned int grid_entries(int64_t n_poses)
     return static_cast<unsigned int>(g);
 }
 
-// Block sums of K values, written by thread 0 to out[k * stride + blockIdx.x].
-// Called at the end of a kernel (its LDS is not used again).
+// Block sums of K values, written by thread 0 to out[k * stride + slot]
+// (slot = blockIdx.x unless given).  Called at the end of a kernel (its LDS
+// is not used again).
 template <int K, int NT = kThreads>
-__device__ inline void write_partials(double (&v)[K], double *out, int stride)
+__device__ inline void write_partials(double (&v)[K], double *out, int stride,
+                                      int slot = -1)
 {
     __shared__ double lds[K][NT / kWave];
     block_sum_last<K, NT>(v, lds);
     if (threadIdx.x == 0) {
+        const int at = slot < 0 ? static_cast<int>(blockIdx.x) : slot;
 #pragma unroll
-        for (int k = 0; k < K; ++k) out[k * stride + blockIdx.x] = v[k];
+        for (int k = 0; k < K; ++k) out[k * stride + at] = v[k];
     }
+}
+
+// The dispatcher deals the blocks of a grid of g round-robin over the XCDs
+// (block b on XCD b % kXcdCount).  Returns b's position when each XCD takes
+// one contiguous range of [0, g) instead, so that neighbouring rows, and the
+// vector lines their gathers share, stay in one XCD's L2.  A bijection on
+// [0, g).
+constexpr int kXcdCount = 8;  // MI355X
+__device__ inline int32_t xcd_contiguous(int32_t b, int32_t g)
+{
+    const int32_t x = b % kXcdCount, k = b / kXcdCount;
+    const int32_t q = g / kXcdCount, r = g % kXcdCount;  // XCDs x < r hold q + 1
+    return x * q + min(x, r) + k;
 }
 
 // acc += part[i] for i = threadIdx.x, +NT, ... < np, in that order, with
