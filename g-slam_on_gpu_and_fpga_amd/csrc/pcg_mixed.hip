@@ -184,7 +184,11 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
     static_assert(kMaxGrid <= 2 * kThreads, "two partials per thread and array");
     constexpr int kRowsPerTile = kThreads / G;
     const int lane = threadIdx.x % G;
-    const int64_t row_first = static_cast<int64_t>(blockIdx.x) * kRowsPerTile + threadIdx.x / G;
+    // row group of this workgroup: in contiguous runs per XCD when one pass
+    // covers all rows (neighbouring rows share an L2), else the round robin
+    const int32_t g = static_cast<int32_t>(gridDim.x), b = static_cast<int32_t>(blockIdx.x);
+    const int32_t bx = (n + kRowsPerTile - 1) / kRowsPerTile <= g ? xcd_contiguous(b, g) : b;
+    const int64_t row_first = static_cast<int64_t>(bx) * kRowsPerTile + threadIdx.x / G;
     // The r.r / r.z partials of this and the previous parity first
     // (np_cur, np_prev <= 2 x kThreads: two per thread and array, summed below
     // in strided_sum's order), as unconditional loads from clamped indices and
@@ -256,7 +260,7 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
     }
     const float beta = k > 0 ? static_cast<float>(red_v[1] / red_v[2]) : 0.f;
     double pq[1] = {0.0};
-    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+    for (int64_t row0 = static_cast<int64_t>(bx) * kRowsPerTile; row0 < n;
          row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
         const int64_t row = row0 + threadIdx.x / G;
         const bool first = row == row_first;
@@ -311,7 +315,7 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
                      static_cast<double>(pi[2]) * y[2];
         }
     }
-    write_partials<1>(pq, pq_out, kSpmvMaxGrid);
+    write_partials<1>(pq, pq_out, kSpmvMaxGrid, bx);  // slot bx: its rows' sum
     if (blockIdx.x == 0 && threadIdx.x == 0) status->iters = k;
 }
 
