@@ -84,14 +84,52 @@ EXPORTS = [
 ]
 
 
+def hip_runtimes() -> list:
+    """Distinct files of the HIP runtime (libamdhip64) mapped in this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            paths = {ln.split(None, 5)[5].strip() for ln in f if "libamdhip64" in ln
+                     and len(ln.split(None, 5)) == 6}
+    except OSError:
+        return []
+    return sorted(os.path.realpath(p) for p in paths)
+
+
+def check_single_runtime():
+    """Raise GSlamError if two HIP runtimes are loaded in this process.
+
+    libgslam.so needs libamdhip64.so.7; PyTorch's ROCm wheels ship their own
+    copy, which their libraries request as libamdhip64.so.  Loaded first,
+    torch's copy satisfies libgslam too (the dynamic linker matches the
+    SONAME libamdhip64.so.7 it carries); loaded after libgslam, it is a second
+    runtime with its own HSA instance, and GPU calls made through one of the
+    two fail (seen as "No HIP GPUs are available" from torch after libgslam
+    had allocated device memory)."""
+    rt = hip_runtimes()
+    if len(rt) > 1:
+        raise GSlamError(ERR_HIP, "load",
+                         "two HIP runtimes are loaded in this process (" + ", ".join(rt) +
+                         "): import torch before loading libgslam.so, or use only one of them")
+
+
 def lib():
-    """Load libgslam.so (once).  Raises GSlamError if it is not built."""
+    """Load libgslam.so (once).  Raises GSlamError if it is not built.
+
+    If PyTorch is installed it is imported first, so that libgslam binds to
+    the HIP runtime torch loads and the process holds one runtime (see
+    check_single_runtime)."""
     global _LIB
     if _LIB is not None:
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise GSlamError(ERR_STATE, "load", f"{LIB_PATH} not built (run __graft_entry__.build())")
+    if os.environ.get("GSLAM_NO_TORCH", "0") != "1":
+        try:
+            import torch  # noqa: F401  (one HIP runtime: torch's, shared with libgslam)
+        except ImportError:
+            pass
     L = C.CDLL(LIB_PATH)
+    check_single_runtime()
     vp = C.c_void_p
     L.gslam_default_opts.argtypes = [vp]
     L.gslam_default_desc.argtypes = [vp]

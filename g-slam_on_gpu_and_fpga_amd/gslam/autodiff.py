@@ -16,7 +16,9 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import Graph
+from . import Graph, check_single_runtime
+
+check_single_runtime()
 
 
 class ConvergedPoses(torch.autograd.Function):
@@ -34,12 +36,16 @@ class ConvergedPoses(torch.autograd.Function):
         kw = dict(solve_kw or {})
         adj_tol = kw.pop("adjoint_tol", 1e-12)
         adj_iter = kw.pop("adjoint_max_iter", 200000)
-        graph.set_log_weights(log_w.detach().to("cpu", torch.float64).numpy())
-        graph.set_lambda(float(lam.detach().cpu()))
+        w = log_w.detach().to("cpu", torch.float64).numpy().copy()
+        lam_v = float(lam.detach().cpu())
+        graph.set_log_weights(w)
+        graph.set_lambda(lam_v)
         graph.set_poses(init_poses)
         graph.converge(**kw)
         x = graph.get_poses()
-        ctx.graph, ctx.x = graph, x
+        # the graph is shared: backward restores this forward's weights, lambda
+        # and converged poses, whatever later forwards left on it
+        ctx.graph, ctx.x, ctx.w, ctx.lam = graph, x, w, lam_v
         ctx.adj = (adj_tol, adj_iter)
         ctx.w_meta = (log_w.dtype, log_w.device)
         ctx.lam_meta = (lam.dtype, lam.device)
@@ -48,6 +54,8 @@ class ConvergedPoses(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_poses):
         g = grad_poses.detach().to("cpu", torch.float64).reshape(-1).numpy()
+        ctx.graph.set_log_weights(ctx.w)
+        ctx.graph.set_lambda(ctx.lam)
         ctx.graph.set_poses(ctx.x)
         dw, dlam, _ = ctx.graph.adjoint(g, tol=ctx.adj[0], max_iter=ctx.adj[1])
         gw = torch.from_numpy(dw).to(dtype=ctx.w_meta[0], device=ctx.w_meta[1])

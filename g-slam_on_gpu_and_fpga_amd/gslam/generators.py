@@ -9,9 +9,11 @@ grid_city: random walk on a unit Manhattan grid (1 m steps, 90 degree turns
   with probability p_turn), odometry noise sigma_xy / sigma_th, loop closures
   between poses closer than lc_radius (true positions) with index gap >
   lc_gap, each kept with probability lc_keep.
-park: smooth random trajectory (bounded curvature, reflecting off the walls of
-  a side x side square), loop closures within lc_radius with gap > lc_gap,
-  subsampled to about lc_per_pose closures per pose.
+park: smooth random trajectory inside a side x side square -- a free walk
+  reflecting off the walls ("wander") or laps of a smooth random closed route
+  with lateral wander ("laps") -- with loop closures within lc_radius and
+  gap > lc_gap, subsampled to lc_per_pose closure edges per pose (fewer when
+  fewer candidate pairs exist).
 
 Measurements are the true relative poses plus Gaussian noise; information is
 diag(1/sigma^2) per edge (lambda and per-edge weights are applied by the
@@ -143,30 +145,75 @@ def grid_city(n_poses: int, seed: int = 0, p_turn: float = 0.2, sigma_xy: float 
     return _finish(truth, I, J, rng, sigma_xy, sigma_th)
 
 
+def _park_laps(n_poses, rng, side, speed, wander):
+    """Laps of a smooth random closed route: r(phi) = R0 + sum_k a_k cos(k phi
+    + phi_k), k = 2..5, around the centre of the square (R0 = side/3, the
+    amplitudes keep the route inside the square), walked at `speed` metres
+    per pose with a smooth lateral offset (an AR(1) process of standard
+    deviation `wander` metres), so that every lap passes within a few metres
+    of the earlier ones -- a jogger on a park path."""
+    r0 = side / 3.0
+    ks = np.arange(2, 6)
+    amp = rng.uniform(0.03, 0.08, len(ks)) * r0
+    ph = rng.uniform(0.0, 2.0 * np.pi, len(ks))
+    fine = np.linspace(0.0, 2.0 * np.pi, 20001)
+    rad = r0 + (amp[:, None] * np.cos(ks[:, None] * fine[None, :] + ph[:, None])).sum(0)
+    pts = np.stack([side / 2 + rad * np.cos(fine), side / 2 + rad * np.sin(fine)], 1)
+    arc = np.concatenate([[0.0], np.cumsum(np.hypot(*np.diff(pts, axis=0).T))])
+    s = (speed * np.arange(n_poses)) % arc[-1]
+    base = np.stack([np.interp(s, arc, pts[:, 0]), np.interp(s, arc, pts[:, 1])], 1)
+    tang = np.stack([np.interp(s, arc, np.gradient(pts[:, 0])),
+                     np.interp(s, arc, np.gradient(pts[:, 1]))], 1)
+    tang /= np.linalg.norm(tang, axis=1, keepdims=True)
+    normal = np.stack([-tang[:, 1], tang[:, 0]], 1)
+    off = np.zeros(n_poses)
+    a = 0.98
+    e = rng.normal(0.0, wander * np.sqrt(1.0 - a * a), n_poses)
+    for k in range(1, n_poses):
+        off[k] = a * off[k - 1] + e[k]
+    xy = np.clip(base + off[:, None] * normal, 0.0, side)
+    d = np.diff(xy, axis=0)
+    th = np.concatenate([np.arctan2(d[:, 1], d[:, 0]), [0.0]])
+    th[-1] = th[-2]
+    return xy, th
+
+
 def park(n_poses: int = 7120, seed: int = 0, side: float = 300.0, speed: float = 1.0,
          sigma_xy: float = 0.1, sigma_th: float = 0.02, lc_radius: float = 3.0,
-         lc_gap: int = 50, lc_per_pose: float = 4.0) -> dict:
-    """Park-like graph: smooth random trajectory inside a side x side square."""
+         lc_gap: int = 50, lc_per_pose: float = 4.0, route: str = "wander",
+         wander: float = 1.0) -> dict:
+    """Park-like graph: smooth random trajectory inside a side x side square.
+
+    route "wander": a free random walk with smoothly varying turn rate that
+    reflects off the walls (revisits are rare: about 1.6 closures per pose
+    survive at 7,120 poses); route "laps": laps of a smooth random closed
+    route with lateral wander (_park_laps), dense enough in revisits that the
+    subsampling reaches lc_per_pose closures per pose."""
     rng = np.random.default_rng(seed)
-    th = np.zeros(n_poses)
-    xy = np.zeros((n_poses, 2))
-    xy[0] = side / 2
-    omega = 0.0
-    for k in range(1, n_poses):
-        omega = 0.9 * omega + rng.normal(0.0, 0.03)
-        t = th[k - 1] + omega
-        p = xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)])
-        # reflect off the walls
-        if p[0] < 0 or p[0] > side:
-            t = np.pi - t
-        if p[1] < 0 or p[1] > side:
-            t = -t
-        p = np.clip(xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)]), 0, side)
-        th[k], xy[k] = t, p
+    if route == "laps":
+        xy, th = _park_laps(n_poses, rng, side, speed, wander)
+    elif route == "wander":
+        th = np.zeros(n_poses)
+        xy = np.zeros((n_poses, 2))
+        xy[0] = side / 2
+        omega = 0.0
+        for k in range(1, n_poses):
+            omega = 0.9 * omega + rng.normal(0.0, 0.03)
+            t = th[k - 1] + omega
+            p = xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)])
+            # reflect off the walls
+            if p[0] < 0 or p[0] > side:
+                t = np.pi - t
+            if p[1] < 0 or p[1] > side:
+                t = -t
+            p = np.clip(xy[k - 1] + speed * np.array([np.cos(t), np.sin(t)]), 0, side)
+            th[k], xy[k] = t, p
+    else:
+        raise ValueError(f"unknown route {route!r}")
     truth = np.concatenate([xy, _wrap(th)[:, None]], 1)
     I, J = _loop_pairs(xy, lc_radius, lc_gap)
     if len(I):
-        want = int(lc_per_pose * n_poses / 2)
+        want = int(lc_per_pose * n_poses)
         keep = rng.permutation(len(I))[: min(want, len(I))]
         keep.sort()
         I, J = I[keep], J[keep]

@@ -19,12 +19,10 @@ DATA = os.path.join(GOLDEN, "data")
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD Instinct MI355X (gfx950)")
     config.addinivalue_line("markers", "slow: longer CPU case")
-    # libgslam needs libamdhip64.so.7 (ROCm's); torch's libraries need
-    # libamdhip64.so and bring their own copy.  Imported after libgslam has
-    # loaded, torch starts a second HIP / HSA runtime in the process (which
-    # failed to find the GPU after a test had made large allocations through
-    # the first); imported before, it loads the runtime libgslam then binds
-    # to.  Import it before any test loads libgslam, whichever files run.
+    # One HIP runtime per process: gslam.lib() imports torch before it loads
+    # libgslam.so (tests/test_runtime.py); importing it here as well keeps
+    # test files that load the library through other routes (the oracle's
+    # ctypes handles, subprocess launches) in the same state.
     try:
         import torch  # noqa: F401
     except ImportError:
