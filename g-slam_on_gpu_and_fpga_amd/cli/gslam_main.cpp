@@ -7,10 +7,29 @@
 //   "Diff: %s Lambda: %s Rounds: %s Drawing: %s Input file: %s"
 //   "delta: %f size_of_us: %d size_of_hat_xs: %d size_of_zlist: %d dim: %d"
 //   "%d rounds executed: %.17g"                       (one per GN round)
-// and, when drawing_mode >= 1 and POSE_DUMP_FILE is set, the final poses as
-// "%u %.17g %.17g %.17g" lines.  Stopping: diff < 0.01, or 20 rounds without
-// a new best diff (GSLAM.c:119-130).  As in the reference GPU driver
-// (src/gpu_pcg_fp64/GSLAM.cu:165-171) the diff is always ||dx||/sqrt(dim).
+// and the final poses as "%u %.17g %.17g %.17g" lines into POSE_DUMP_FILE.
+//
+// PCG solvers (pcg64, mixed) follow the reference GPU PCG driver
+// (src/gpu_pcg_fp64/GSLAM.cu:150-200): diff = ||dx||/sqrt(dim) whatever
+// use_diff says, stop at diff < 0.01 or after 20 rounds without a new best
+// diff (src/cpu_pcg_fp64/GSLAM.c:119-130), pose dump when drawing_mode >= 1.
+//
+// Gauss-Jordan solvers (gj64, gj64b, gjmixed) follow the reference GPU GJ
+// drivers (src/gpu_gj_fp64/GSLAM.cu:193-591, the same in gpu_gj_mixed):
+//   use_diff 1: diff = ||dx||/sqrt(dim); stop at diff < 0.01, or once the
+//     even rounds or the odd rounds have gone 10 of their own rounds without
+//     a new best (:518-530, the 2-cycle plateau);
+//   use_diff 0: diff = |cost - last_cost| / cost with the running cost of
+//     gslam_obs_cost (never reset, :245-257, 454); stop at diff < 1e-4,
+//     otherwise print "last_cost: %.17g" and "cost: %.17g" (:533-591);
+// and the pose dump is written whenever POSE_DUMP_FILE is set (:757-770).
+//
+// Pose graphs: GSLAM_BETWEEN=<file> adds SE(2) between edges from a sidecar
+// file (lines "e i j dx dy dth info_x info_y info_th", '#' comments; i, j
+// index the x records in file order), kept out of the .log.txt because the
+// reference reader takes any unknown first token as a delta record
+// (src/cpu_pcg_fp64/GSLAM.c:480-483).  With a sidecar and no u records, no
+// motion edges are formed.
 //
 // Extra knobs (environment, defaults reproduce the reference):
 //   GSLAM_SOLVER   pcg64 | mixed | gj64 | gj64b | gjmixed  (pcg64)
@@ -85,6 +104,39 @@ int read_log(const char *path, LogData &d)
     return 0;
 }
 
+// Between-edge sidecar: "e i j dx dy dth info_x info_y info_th" per line.
+int read_between(const char *path, std::vector<int32_t> &bi, std::vector<int32_t> &bj,
+                 std::vector<double> &meas, std::vector<double> &info)
+{
+    FILE *f = std::fopen(path, "r");
+    if (!f) {
+        std::fprintf(stderr, "GSLAM: cannot open between-edge file %s\n", path);
+        return -1;
+    }
+    char line[1024];
+    int lineno = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++lineno;
+        const char *c = line;
+        while (*c == ' ' || *c == '\t') ++c;
+        if (*c == '#' || *c == '\n' || *c == '\r' || *c == '\0') continue;
+        int i = 0, j = 0;
+        double v[6];
+        if (std::sscanf(c, "e %d %d %lf %lf %lf %lf %lf %lf", &i, &j, &v[0], &v[1], &v[2], &v[3],
+                        &v[4], &v[5]) != 8) {
+            std::fprintf(stderr, "GSLAM: malformed between edge at %s:%d\n", path, lineno);
+            std::fclose(f);
+            return -2;
+        }
+        bi.push_back(i);
+        bj.push_back(j);
+        meas.insert(meas.end(), v, v + 3);
+        info.insert(info.end(), v + 3, v + 6);
+    }
+    std::fclose(f);
+    return 0;
+}
+
 double env_double(const char *name, double dflt)
 {
     const char *v = std::getenv(name);
@@ -127,7 +179,12 @@ int main(int argc, char *argv[])
     std::fflush(stdout);
 
     const int n = static_cast<int>(d.x_step.size());
-    if (n == 0 || d.u_nu.size() != d.x_step.size()) {
+    std::vector<int32_t> be_i, be_j;
+    std::vector<double> be_meas, be_info;
+    const char *between = std::getenv("GSLAM_BETWEEN");
+    if (between && read_between(between, be_i, be_j, be_meas, be_info) != 0) return 1;
+    const bool motion = !(between && d.u_nu.empty());
+    if (n == 0 || (motion && d.u_nu.size() != d.x_step.size())) {
         std::fprintf(stderr, "GSLAM: need one u record per x record (u=%zu, x=%zu)\n",
                      d.u_nu.size(), d.x_step.size());
         return 1;
@@ -138,9 +195,9 @@ int main(int argc, char *argv[])
     gslam_default_desc(&desc);
     desc.n_poses = n;
     desc.poses = d.poses.data();
-    desc.motion = 1;
-    desc.u_nu = d.u_nu.data();
-    desc.u_omega = d.u_omega.data();
+    desc.motion = motion ? 1 : 0;
+    desc.u_nu = motion ? d.u_nu.data() : nullptr;
+    desc.u_omega = motion ? d.u_omega.data() : nullptr;
     desc.delta = d.delta;
     desc.lambda = lambda;
     desc.n_obs = static_cast<int32_t>(zs.size());
@@ -150,6 +207,11 @@ int main(int argc, char *argv[])
     desc.z_bearing = d.z_bearing.data();
     desc.pair_order = env_int("GSLAM_ORDER", 0);
     desc.wrap = env_int("GSLAM_WRAP", 0);
+    desc.n_between = static_cast<int64_t>(be_i.size());
+    desc.be_i = be_i.data();
+    desc.be_j = be_j.data();
+    desc.be_meas = be_meas.data();
+    desc.be_info = be_info.data();
 
     gslam_graph *g = nullptr;
     int rc = gslam_create(&desc, &g);
@@ -171,9 +233,23 @@ int main(int argc, char *argv[])
     opts.max_iter = env_int("GSLAM_MAX_ITER", opts.max_iter);
     const bool stats_json = env_int("GSLAM_STATS", 0) != 0;
 
-    double best = 1e30;
-    unsigned int no_improve = 0;
+    const bool gj = opts.solver == GSLAM_SOLVER_GJ_FP64 || opts.solver == GSLAM_SOLVER_GJ_MIXED ||
+                    opts.solver == GSLAM_SOLVER_GJ_BLOCKED64;
+    const bool use_diff = std::atoi(argv[1]) != 0;
+    double best = 1e30, best_even = 1e30, best_odd = 1e30;
+    unsigned int no_improve = 0, no_improve_even = 0, no_improve_odd = 0;
+    double cost = 0.0, last_cost = 0.0;
     for (unsigned int r = 1; r <= rounds; ++r) {
+        // GJ drivers, use_diff 0: the cost is taken before the solve, at the
+        // poses this round linearises at (GSLAM.cu:245-257)
+        if (gj && !use_diff) {
+            rc = gslam_obs_cost(g, &cost);
+            if (rc != GSLAM_OK) {
+                std::fprintf(stderr, "GSLAM: cost failed (%d): %s\n", rc, gslam_last_error());
+                gslam_destroy(g);
+                return 1;
+            }
+        }
         gslam_solve_stats st;
         double diff = 0.0;
         rc = gslam_gn_step(g, &opts, &st, &diff);
@@ -182,6 +258,7 @@ int main(int argc, char *argv[])
             gslam_destroy(g);
             return 1;
         }
+        if (gj && !use_diff) diff = std::fabs(cost - last_cost) / cost;
         std::printf("%d rounds executed: %.17g\n", r, diff);
         std::fflush(stdout);
         if (stats_json)
@@ -189,15 +266,35 @@ int main(int argc, char *argv[])
                          "{\"round\": %u, \"iters\": %d, \"converged\": %d, \"rel_res\": %.6e, "
                          "\"true_rel_res\": %.6e, \"solve_ms\": %.4f}\n",
                          r, st.iters, st.converged, st.rel_res, st.true_rel_res, st.ms);
-        if (diff < 0.01) break;
-        if (diff < best) {
-            best = diff;
-            no_improve = 0;
-        } else if (++no_improve >= 20) {
-            break;
+        if (gj && use_diff) {
+            if (diff < 0.01) break;
+            // separate plateaus for even and odd rounds (a 2-cycle of the
+            // iteration does not stop the run by itself)
+            double &b = (r % 2 == 0) ? best_even : best_odd;
+            unsigned int &k = (r % 2 == 0) ? no_improve_even : no_improve_odd;
+            if (diff < b) {
+                b = diff;
+                k = 0;
+            } else if (++k >= 10) {
+                break;
+            }
+        } else if (gj) {
+            if (diff < 1e-4) break;
+            std::printf("last_cost: %.17g\n", last_cost);
+            std::printf("cost: %.17g\n", cost);
+            std::fflush(stdout);
+            last_cost = cost;
+        } else {
+            if (diff < 0.01) break;
+            if (diff < best) {
+                best = diff;
+                no_improve = 0;
+            } else if (++no_improve >= 20) {
+                break;
+            }
         }
     }
-    if (draw >= 1) {
+    if (draw >= 1 || gj) {
         const char *dump = std::getenv("POSE_DUMP_FILE");
         if (dump) {
             std::vector<double> poses(3 * static_cast<size_t>(n));

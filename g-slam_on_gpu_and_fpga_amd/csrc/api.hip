@@ -259,6 +259,16 @@ int gslam_get_poses(gslam_graph *gh, double *poses)
     return g.download(poses, g.poses, 3 * static_cast<int64_t>(g.h.n) * 8);
 }
 
+int gslam_obs_cost(gslam_graph *gh, double *cost)
+{
+    if (!gh || !cost) return GSLAM_ERR_ARG;
+    DeviceGraph &g = gh->g;
+    std::vector<double> poses(3 * static_cast<size_t>(g.h.n));
+    GSLAM_TRY(g.download(poses.data(), g.poses, 3 * static_cast<int64_t>(g.h.n) * 8));
+    *cost = obs_cost_terms(g.h, poses.data(), g.snr, *cost);
+    return GSLAM_OK;
+}
+
 int gslam_set_lambda(gslam_graph *gh, double lambda)
 {
     if (!gh) return GSLAM_ERR_ARG;

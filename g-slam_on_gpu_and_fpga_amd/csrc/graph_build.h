@@ -67,4 +67,9 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err);
 void landmark_pairs(int32_t n_obs, const int32_t *z_lm, int order,
                     std::vector<int32_t> &a, std::vector<int32_t> &b);
 
+// acc + the landmark-pair terms of the reference Gauss-Jordan drivers' cost
+// (src/gpu_gj_fp64/GSLAM.cu:245-257) at `poses` (3n doubles), summed onto
+// acc in edge order (reference_cost.cpp).
+double obs_cost_terms(const HostGraph &h, const double *poses, const double snr[2], double acc);
+
 }  // namespace gslam

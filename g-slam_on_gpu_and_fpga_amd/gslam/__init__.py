@@ -77,7 +77,7 @@ EXPORTS = [
     "gslam_set_log_weights", "gslam_assemble", "gslam_get_system_csr", "gslam_get_system_bsr",
     "gslam_solve", "gslam_get_dx", "gslam_solve_rhs", "gslam_apply_update", "gslam_gn_step",
     "gslam_gj_invert", "gslam_last_error", "gslam_set_system_csr", "gslam_time_kernel",
-    "gslam_adjoint",
+    "gslam_adjoint", "gslam_obs_cost",
     "device_graph_init", "device_graph_update", "device_graph_get_row_ptr",
     "device_graph_get_col_idx", "device_build_diag_inv", "device_pcg_solve",
     "device_graph_free", "inv_cuda",
@@ -152,6 +152,7 @@ def lib():
     L.gslam_last_error.restype = C.c_char_p
     L.gslam_set_system_csr.argtypes = [vp, vp, vp]
     L.gslam_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, vp]
+    L.gslam_obs_cost.argtypes = [vp, vp]
     _LIB = L
     return L
 
@@ -412,6 +413,13 @@ class Graph:
         _check(lib().gslam_solve_rhs(self.h, C.byref(o), _p(b), _p(x), C.byref(s)),
                "gslam_solve_rhs", allow=(ERR_NOT_CONVERGED, ERR_SINGULAR))
         return x, _stats(s)
+
+    def obs_cost(self, acc: float = 0.0) -> float:
+        """acc + the reference GJ drivers' use_diff=0 cost terms at the
+        current poses (gslam_obs_cost)."""
+        c = C.c_double(acc)
+        _check(lib().gslam_obs_cost(self.h, C.byref(c)), "gslam_obs_cost")
+        return c.value
 
     def apply_update(self) -> float:
         diff = C.c_double()

@@ -224,3 +224,22 @@ def graph_kwargs(g: dict, lam: float, wrap: int = 1) -> dict:
     """Keyword arguments of gslam.Graph for a generated pose graph."""
     return dict(lam=lam, between=dict(i=g["i"], j=g["j"], meas=g["meas"], info=g["info"]),
                 wrap=wrap)
+
+
+def write_pose_graph(g: dict, log_path: str, between_path: str) -> None:
+    """Writes a generated pose graph for the GSLAM command-line driver: the
+    initial poses as .log.txt x records ("x <i> <x> <y> <theta>", the record
+    src/cpu_pcg_fp64/GSLAM.c:450-466 reads) and the between edges as the
+    sidecar GSLAM_BETWEEN reads ("e <i> <j> <dx> <dy> <dth> <info_x> <info_y>
+    <info_th>").  Values as %.17g, so they read back to the same doubles."""
+    p = np.asarray(g["poses"]).reshape(-1, 3)
+    with open(log_path, "w") as f:
+        for k, (x, y, t) in enumerate(p):
+            f.write(f"x {k} {x:.17g} {y:.17g} {t:.17g}\n")
+    m = np.asarray(g["meas"]).reshape(-1, 3)
+    inf = np.asarray(g["info"]).reshape(-1, 3)
+    with open(between_path, "w") as f:
+        f.write("# SE(2) between edges: e i j dx dy dth info_x info_y info_th\n")
+        for i, j, a, b in zip(g["i"], g["j"], m, inf):
+            f.write(f"e {int(i)} {int(j)} {a[0]:.17g} {a[1]:.17g} {a[2]:.17g} "
+                    f"{b[0]:.17g} {b[1]:.17g} {b[2]:.17g}\n")

@@ -202,6 +202,16 @@ int gslam_solve_rhs(gslam_graph *g, const gslam_solve_opts *opts,
  * diff = ||dx|| / sqrt(3N) (GSLAM.c:104-105). */
 int gslam_apply_update(gslam_graph *g, double *diff_out);
 
+/* Cost of the reference Gauss-Jordan drivers' use_diff = 0 stopping test
+ * (src/gpu_gj_fp64/GSLAM.cu:245-257; src/cpu_gj_mixed/GSLAM.c:74-86) at the
+ * current poses: for every landmark-pair edge, in edge order, the stored
+ * residual e and 2x2 information inv(Sigma) (ObsEdge.c:76-82, 126-140) read
+ * through the drivers' 3x3 view, e^T M e, added to *cost (motion edges add
+ * zero).  *cost is the running sum the caller keeps: the reference never
+ * resets it between rounds.  Host arithmetic in the reference's order (the
+ * same bits as the reference's printf("cost: %.17g")). */
+int gslam_obs_cost(gslam_graph *g, double *cost);
+
 /* assemble + solve + apply_update. */
 int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
                   gslam_solve_stats *stats, double *diff_out);

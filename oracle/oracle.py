@@ -264,6 +264,36 @@ class Oracle:
         self.L.orc_dense_matvec(a.shape[0], _ptr(a), _ptr(x), _ptr(y))
         return y
 
+    def gj_cost(self, g: dict, poses=None, order: int = 0, acc: float = 0.0) -> float:
+        """acc + the cost terms of the reference GJ drivers' use_diff=0 test
+        (src/gpu_gj_fp64/GSLAM.cu:245-257, src/cpu_gj_mixed/GSLAM.c:74-86):
+        per landmark-pair edge in make_edges order, the stored residual e
+        (2 values) and information Om = inv(Sigma) (2x2, flattened into the
+        first 4 of 9 slots, ObsEdge.c:76-82, 126-140, 200-207) read as a
+        3-vector and a 3x3 matrix; e^T M e summed from 0.0 in the reference's
+        dot order and added to the running cost.  Motion edges store e = 0
+        and add 0.  The third component of e lies past the array (the next
+        edge record's first bytes, a subnormal): taken as 0."""
+        poses = np.ascontiguousarray(g["poses"] if poses is None else poses, np.float64)
+        pa, pb = self.obs_pairs(g["z_lm"], order)
+        zs, zr, zb = g["z_step"], g["z_range"], g["z_bearing"]
+        for a, b in zip(pa, pb):
+            _, e, om = self.obs_edge(poses[3 * zs[a]:3 * zs[a] + 3], poses[3 * zs[b]:3 * zs[b] + 3],
+                                     float(zr[a]), float(zb[a]), float(zr[b]), float(zb[b]))
+            ev = (float(e[0]), float(e[1]), 0.0)
+            M = (float(om[0]), float(om[1]), float(om[2]), float(om[3]), 0.0, 0.0, 0.0, 0.0, 0.0)
+            v = []
+            for j in range(3):
+                t = 0.0
+                for k in range(3):
+                    t += ev[k] * M[3 * k + j]
+                v.append(t)
+            s = 0.0
+            for k in range(3):
+                s += v[k] * ev[k]
+            acc += s * 1.0
+        return acc
+
     def landmark_means(self, g: dict, poses=None):
         """(ids, means[L, 2]) of the landmarks, GSLAM.c:141-196."""
         poses = np.ascontiguousarray(g["poses"] if poses is None else poses, np.float64)
@@ -436,6 +466,36 @@ class RefPCG64:
         self.L.make_edges(hx, len(poses) // 3, zl, nz, edges, C.byref(ne), lkz, C.byref(nl))
         return edges, ne.value, hx
 
+    def gj_cost(self, g: dict, poses=None, order: int = 0, acc: float = 0.0) -> float:
+        """acc + the cost terms of the reference GJ drivers' use_diff=0 test
+        (src/gpu_gj_fp64/GSLAM.cu:245-257, src/cpu_gj_mixed/GSLAM.c:74-86):
+        per landmark-pair edge in make_edges order, the stored residual e
+        (2 values) and information Om = inv(Sigma) (2x2, flattened into the
+        first 4 of 9 slots, ObsEdge.c:76-82, 126-140, 200-207) read as a
+        3-vector and a 3x3 matrix; e^T M e summed from 0.0 in the reference's
+        dot order and added to the running cost.  Motion edges store e = 0
+        and add 0.  The third component of e lies past the array (the next
+        edge record's first bytes, a subnormal): taken as 0."""
+        poses = np.ascontiguousarray(g["poses"] if poses is None else poses, np.float64)
+        pa, pb = self.obs_pairs(g["z_lm"], order)
+        zs, zr, zb = g["z_step"], g["z_range"], g["z_bearing"]
+        for a, b in zip(pa, pb):
+            _, e, om = self.obs_edge(poses[3 * zs[a]:3 * zs[a] + 3], poses[3 * zs[b]:3 * zs[b] + 3],
+                                     float(zr[a]), float(zb[a]), float(zr[b]), float(zb[b]))
+            ev = (float(e[0]), float(e[1]), 0.0)
+            M = (float(om[0]), float(om[1]), float(om[2]), float(om[3]), 0.0, 0.0, 0.0, 0.0, 0.0)
+            v = []
+            for j in range(3):
+                t = 0.0
+                for k in range(3):
+                    t += ev[k] * M[3 * k + j]
+                v.append(t)
+            s = 0.0
+            for k in range(3):
+                s += v[k] * ev[k]
+            acc += s * 1.0
+        return acc
+
     def landmark_means(self, g: dict, poses=None):
         """The reference's post-loop (GSLAM.c:141-196) through its own
         make_edges, MapEdge_create and vec_mean: (ids, means[L, 2])."""
@@ -532,7 +592,12 @@ class RefPCG64:
 
 
 class RefGJMixed:
-    """Build of src/cpu_gj_mixed (C++): inv_float, MatrixOps.c:174-212."""
+    """Build of src/cpu_gj_mixed (C++): inv_float, MatrixOps.c:174-212, and
+    the executable (its GN driver, GSLAM.c:28-250)."""
+
+    @staticmethod
+    def exe():
+        return os.path.join(REF_DIR, "GSLAM_cpu_gj_mixed")
 
     def __init__(self):
         L = C.CDLL(os.path.join(REF_DIR, "libref_cpu_gj_mixed.so"))
