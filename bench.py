@@ -9,12 +9,15 @@ accuracy check applies to each of them.
 
 Headline workload: synthetic grid-city pose graph (gslam.generators.grid_city),
 N = 100,000 poses, seed 0, lambda 0.1, angle wrap on.  Accuracy: relative dx
-error against an FP64 PCG solve of the same system at tolerance 1e-14 (the
-attainable floor: solves at 1e-14 / 3e-14 / 1e-13 agree to 1-6e-9).  The FP64
-solver runs at tolerance 3e-13, the loosest that meets rel dx err <= 1e-8 on
-this graph (1e-12 gives 1.1-1.3e-8); the mixed solver runs at 1e-12, the
-tightest its FP64 true-residual refreshes certify here, and is reported with
-its accuracy.
+error against an independent direct solve of the same system (SuperLU,
+refined on residuals in x87 extended precision: ~1e-11 relative; the FP64
+LU solve alone is 1.3e-7 away), with the FP64 PCG solve at tolerance 1e-14
+reported beside it.  The FP64 solver runs at tolerance 3e-13, the loosest
+that meets rel dx err <= 1e-8 on this graph (1e-12 gives 1.3e-8); the mixed
+solver runs FP32 iterations until the FP64 true residual is below 1e-4 and
+then FP64 PCG on the correction to the same 3e-13 (FP32 iterations alone
+stall at ~4e-7 error: kappa ~ 1e13 of the block-Jacobi preconditioned
+operator is far beyond 1/eps32).
 
 Also reported:
 * roofline: the FP64 BSR SpMV's mean per-launch duration inside the timed
@@ -30,6 +33,8 @@ Also reported:
   FP64 (GJ_BLOCKED64) against the plain FP64 Gauss-Jordan and
   numpy.linalg.solve (FP64), grid-city N=800, 1600 and 5000 (3N = 2400,
   4800, 15000; the plain FP64 Gauss-Jordan up to N=2000);
+* park: the park graph (laps route) of 7,120 poses, lambda 1.0 -- FP64 and
+  mixed PCG at the same tolerance against the same kind of direct solve;
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -85,6 +90,31 @@ def measured_traffic():
         return None
 
 
+def refined_direct(rp, ci, va, xi, steps=4):
+    """Direct solve of the CSR system: SuperLU (FP64) refined with residuals
+    formed in x87 extended precision (numpy.longdouble) -- accurate to ~1e-11
+    relative on the N=100k grid-city system."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = len(rp) - 1
+    lu = spla.splu(sp.csr_matrix((va, ci, rp), shape=(n, n)).tocsc())
+    Al = sp.csr_matrix((va.astype(np.longdouble), ci, rp), shape=(n, n))
+    x = lu.solve(xi).astype(np.longdouble)
+    bl = xi.astype(np.longdouble)
+    for _ in range(steps):
+        x += lu.solve((bl - Al @ x).astype(np.float64))
+    return x.astype(np.float64)
+
+
+def accuracy(np, dx, dx_ref):
+    err = float(np.linalg.norm(dx - dx_ref) / np.linalg.norm(dx_ref))
+    d3 = (dx - dx_ref).reshape(-1, 3)
+    return dict(rel_dx_err=err, meets_1e8=err <= 1e-8, max_abs_dx_err=float(np.abs(d3).max()),
+                max_pose_err_m=float(np.abs(d3[:, :2]).max()),
+                max_pose_err_rad=float(np.abs(d3[:, 2]).max()))
+
+
 def hip_sync():
     import ctypes
     ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
@@ -98,15 +128,20 @@ def pcg_section(args, gslam, G, np):
     g = gslam.Graph(init, **G.graph_kwargs(gc, 0.1, wrap=1))
     info = g.info
     log(f"handle: {info}")
-    dx_ref = None
+    dx_ref = dx_pcg = None
     if not args.no_accuracy:
         g.set_poses(init)
         g.assemble()
+        t = time.time()
+        dx_ref = refined_direct(*g.system_csr())
+        log(f"reference: refined direct solve on the host ({time.time() - t:.1f} s)")
         st = g.solve("pcg64", args.ref_tol, args.max_iter)
-        dx_ref = g.dx()
-        log(f"reference solve (tol {args.ref_tol:g}): {st['iters']} its, "
-            f"true rel res {st['true_rel_res']:.2e}")
+        dx_pcg = g.dx()
+        log(f"FP64 PCG at tol {args.ref_tol:g}: {st['iters']} its, true rel res "
+            f"{st['true_rel_res']:.2e}, rel dx err vs the direct solve "
+            f"{np.linalg.norm(dx_pcg - dx_ref) / np.linalg.norm(dx_ref):.2e}")
     tols = {"pcg64": args.tol, "mixed": args.mixed_tol}
+    kws = {"pcg64": {}, "mixed": dict(mixed_handover=args.handover)}
     results, samples = {}, []
     for solver in [s for s in args.solvers.split(",") if s]:
         tol = tols[solver]
@@ -114,26 +149,26 @@ def pcg_section(args, gslam, G, np):
         if dx_ref is not None:
             g.set_poses(init)
             g.assemble()
-            st = g.solve(solver, tol, args.max_iter)
+            st = g.solve(solver, tol, args.max_iter, **kws[solver])
             dx = g.dx()
-            err = float(np.linalg.norm(dx - dx_ref) / np.linalg.norm(dx_ref))
-            d3 = (dx - dx_ref).reshape(-1, 3)
-            acc = dict(rel_dx_err=err, meets_1e8=err <= 1e-8, max_abs_dx_err=float(np.abs(d3).max()),
-                       max_pose_err_m=float(np.abs(d3[:, :2]).max()),
-                       max_pose_err_rad=float(np.abs(d3[:, 2]).max()),
+            acc = accuracy(np, dx, dx_ref)
+            acc.update(rel_dx_err_vs_pcg_1e14=float(np.linalg.norm(dx - dx_pcg) / np.linalg.norm(dx_pcg)),
                        true_rel_res=st["true_rel_res"], converged=st["converged"],
-                       refreshes=st["reliable_updates"])
+                       refreshes=st["reliable_updates"], fp64_correction_iters=st["iters_fp64"])
+            if solver == "mixed":
+                acc["handover"] = args.handover
             log(f"{solver}: accuracy {acc}")
         for _ in range(args.warmup):
             g.set_poses(init)
-            g.gn_step(solver, tol, args.max_iter)
+            g.gn_step(solver, tol, args.max_iter, **kws[solver])
         hip_sync()
         iters = []
         t_start = time.perf_counter()
         for _ in range(args.steps):
             g.set_poses(init)
             st, _ = g.gn_step(solver, tol, args.max_iter,
-                              profile_every=args.profile_every if solver == "pcg64" else 0)
+                              profile_every=args.profile_every if solver == "pcg64" else 0,
+                              **kws[solver])
             iters.append(st["iters"])
             if solver == "pcg64" and st["spmv_us"] > 0:
                 samples.append((st["spmv_us"], st["update_us"]))
@@ -150,6 +185,10 @@ def pcg_section(args, gslam, G, np):
         us = float(np.mean([s[0] for s in samples]))
         achieved = byts / (us * 1e-6) / 1e9
         roofline = dict(bound="hbm", kernel="pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)",
+                        regime=("latency-bound: the 31 MB matrix and the vectors stay in the "
+                                "256 MB Infinity Cache at N=100k; the fraction is algorithmic "
+                                "bytes over time against the 8 TB/s HBM peak (counted HBM "
+                                "traffic in 'traffic')"),
                         us_per_launch=us, bytes_per_launch=byts, achieved=achieved,
                         peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                         traffic=measured_traffic(),
@@ -271,7 +310,7 @@ def gj_section(args, gslam, G, np):
             nx = np.linalg.norm(x_np)
             row = dict(n=n, numpy_solve_s=t_np)
             for name, kw in (("ir3", dict(refine_sweeps=3, refine_mode="ir", tol=0.0)),
-                             ("cg", dict(refine_sweeps=200, refine_mode="cg", tol=1e-13))):
+                             ("cg", dict(refine_sweeps=400, refine_mode="cg", tol=2e-12))):
                 g.solve("gj_mixed", **kw)   # warm-up (work space allocation)
                 st = g.solve("gj_mixed", **kw)
                 e = float(np.linalg.norm(g.dx() - x_np) / nx)
@@ -369,6 +408,87 @@ def adjoint_section(args, gslam, G, np):
     return out
 
 
+def park_section(args, gslam, G, np):
+    """C4b: FP64 and mixed PCG on the park graph at one tolerance, each with
+    its error against the refined direct solve; ms of the solve alone (HIP
+    events) and of a whole GN step."""
+    gc = G.park(args.park_n, seed=0, route="laps")
+    out = dict(n_poses=args.park_n, n_loop_closures=int(gc["n_loop"]), lambda_=1.0,
+               tol=args.park_tol)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 1.0, wrap=1)) as g:
+        g.assemble()
+        dx_ref = refined_direct(*g.system_csr())
+        for name, kw in (("pcg64", {}), ("mixed", dict(mixed_handover=args.park_handover))):
+            g.set_poses(gc["poses"])
+            g.assemble()
+            g.solve(name, args.park_tol, 400000, **kw)
+            st = g.solve(name, args.park_tol, 400000, **kw)
+            acc = accuracy(np, g.dx(), dx_ref)
+            hip_sync()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                g.set_poses(gc["poses"])
+                g.gn_step(name, args.park_tol, 400000, **kw)
+            hip_sync()
+            ms = 1000.0 * (time.perf_counter() - t) / args.steps
+            out[name] = dict(ms_per_step=ms, solve_ms=st["ms"], pcg_iters=st["iters"],
+                             fp64_correction_iters=st["iters_fp64"], **acc, **kw)
+            log(f"park {name}: {ms:.1f} ms per GN step (solve {st['ms']:.1f} ms, "
+                f"{st['iters']} its), rel dx err {acc['rel_dx_err']:.2e}")
+    out["mixed_speedup_vs_fp64"] = out["pcg64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    return out
+
+
+def gj_cpu_baseline(args, np):
+    """The reference's FP64 Gauss-Jordan inv() (src/cpu_pcg_fp64/MatrixOps.c:29-91,
+    oracle/_ref) on the host, one thread, at n = args.gj_cpu_n on the N=800
+    grid-city Omega's leading block (its work is 4 n^3 flops as written), and
+    scaled by (2400 / n)^3 to the N=800 system (n = 2400)."""
+    if args.gj_cpu_n <= 0:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        from oracle import Oracle, RefPCG64, ref_available
+    except ImportError:
+        return None
+    if not ref_available():
+        return None
+    import threading
+    import scipy.sparse as sp
+    from gslam import generators as G
+    gc = G.grid_city(800, seed=0)
+    O = Oracle()
+    ev = O.between_edges(gc["poses"], gc["i"], gc["j"], gc["meas"], gc["info"], None, 0.1, 1)
+    rp, ci, va, _ = O.assemble_csr(800, gc["i"], gc["j"], ev)
+    n = args.gj_cpu_n
+    A = sp.csr_matrix((va, ci, rp), shape=(2400, 2400)).toarray()[:n, :n].copy()
+    ref = RefPCG64()
+    box = {}
+
+    def run():
+        t = time.perf_counter()
+        box["inv"] = ref.inv(A)
+        box["s"] = time.perf_counter() - t
+
+    # inv() keeps an n x n VLA on the stack: run it on a thread with room for it
+    old = threading.stack_size(max(64 << 20, 8 * n * n + (16 << 20)))
+    try:
+        th = threading.Thread(target=run)
+        th.start()
+        th.join()
+    finally:
+        threading.stack_size(old)
+    s = box["s"]
+    err = float(np.abs(box["inv"] @ A - np.eye(n)).max())
+    log(f"reference inv() n={n}: {s:.2f} s on one host thread (|A inv - I| {err:.1e}); "
+        f"x{(2400 / n) ** 3:.0f} for n=2400")
+    return dict(value=1000.0 * s * (2400.0 / n) ** 3, unit="ms per GJ inversion (n=2400)", cores=1,
+                kind="reference",
+                sample=f"reference inv() (oracle/_ref build of src/cpu_pcg_fp64/MatrixOps.c) on "
+                       f"the leading {n} x {n} block of the N=800 grid-city Omega: {s:.2f} s, "
+                       f"scaled by (2400/{n})^3", measured_s=s, n=n)
+
+
 def rcm_relabel(gc):
     """The grid-city graph with its poses relabelled in reverse Cuthill-McKee
     order of the edge graph (scipy), so that edges join nearby labels."""
@@ -450,7 +570,15 @@ def main():
                     help="pose labels for --profile-kernel spmv64/update64: generator order "
                          "or reverse Cuthill-McKee")
     ap.add_argument("--tol", type=float, default=3e-13, help="FP64 PCG tolerance")
-    ap.add_argument("--mixed-tol", type=float, default=1e-12)
+    ap.add_argument("--mixed-tol", type=float, default=3e-13)
+    ap.add_argument("--handover", type=float, default=1e-4,
+                    help="mixed: FP64 true relative residual at which FP64 PCG takes over the "
+                         "correction (0: FP32 iterations all the way)")
+    ap.add_argument("--park-n", type=int, default=7120, help="park section size (0 skips)")
+    ap.add_argument("--park-tol", type=float, default=1e-9)
+    ap.add_argument("--park-handover", type=float, default=1e-2)
+    ap.add_argument("--gj-cpu-n", type=int, default=1200,
+                    help="dimension of the reference inv() CPU sample (0 skips)")
     ap.add_argument("--ref-tol", type=float, default=1e-14)
     ap.add_argument("--max-iter", type=int, default=600000)
     ap.add_argument("--solvers", default="mixed,pcg64")
@@ -488,6 +616,10 @@ def main():
     gc, init, info, results, roofline, kernels = pcg_section(args, gslam, G, np)
     cpu = cpu_section(args, gc, init, info, results)
     gj = gj_section(args, gslam, G, np) if args.gj_n else None
+    park = park_section(args, gslam, G, np) if args.park_n > 0 else None
+    gj_cpu = gj_cpu_baseline(args, np)
+    if cpu is not None and gj_cpu is not None:
+        cpu["gj_inv"] = gj_cpu
     adj = adjoint_section(args, gslam, G, np) if args.adj_n > 0 else None
 
     fp = results.get("pcg64", {})
@@ -508,7 +640,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"grid-city pose graph, N={info['n_poses']} poses, seed {args.seed}, "
                                f"lambda 0.1, one GN step from the dead-reckoned initial guess, "
-                               f"block-Jacobi PCG to rel tol {args.tol:g} (FP64)",
+                               f"block-Jacobi PCG to rel tol {args.tol:g} (FP64); mixed: FP32 "
+                               f"iterations to {args.handover:g}, then FP64 to {args.mixed_tol:g}",
                    "n_poses": info["n_poses"], "n_edges": info["n_edges"],
                    "n_blocks": info["n_blocks"], "nnz": info["nnz"],
                    "headline_solver": "pcg64" if fp else "mixed"},
@@ -519,6 +652,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gj": gj,
+        "park": park,
         "adjoint": adj,
     }
     print(json.dumps(out), flush=True)

@@ -37,6 +37,8 @@
 //   GSLAM_REFINE   gjmixed refinement: ir | cg (ir)
 //   GSLAM_SWEEPS   gjmixed refinement iterations (3)
 //   GSLAM_MAX_ITER PCG iteration cap           (500,  MatrixOps.c:347)
+//   GSLAM_HANDOVER mixed: true relative residual at which FP64 PCG takes over
+//                  the correction (0: FP32 iterations all the way)
 //   GSLAM_WRAP     1: wrap theta residuals and poses (0)
 //   GSLAM_ORDER    landmark pair order 0 (CPU reference) | 1 (GPU reference)
 //   GSLAM_STATS    1: one JSON line per round on stderr
@@ -231,6 +233,7 @@ int main(int argc, char *argv[])
     opts.refine_sweeps = env_int("GSLAM_SWEEPS", opts.refine_sweeps);
     opts.tol = env_double("GSLAM_TOL", opts.tol);
     opts.max_iter = env_int("GSLAM_MAX_ITER", opts.max_iter);
+    opts.mixed_handover = env_double("GSLAM_HANDOVER", opts.mixed_handover);
     const bool stats_json = env_int("GSLAM_STATS", 0) != 0;
 
     const bool gj = opts.solver == GSLAM_SOLVER_GJ_FP64 || opts.solver == GSLAM_SOLVER_GJ_MIXED ||

@@ -86,10 +86,8 @@ extern "C" int gslam_adjoint(gslam_graph *gh, const double *dl_dx, const gslam_s
         return GSLAM_ERR_ARG;
     }
     const int64_t dim = 3 * static_cast<int64_t>(h.n), nb = h.n_bet_e;
-    double *b = nullptr, *v = nullptr, *gw = nullptr;
-    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&b), static_cast<size_t>(2 * dim + nb + 1) * 8));
-    v = b + dim;
-    gw = v + dim;
+    GSLAM_TRY(g.vecs_reserve());
+    double *b = g.vecs, *v = g.vecs + dim, *gw = g.vecs + 4 * dim;
     gslam_solve_opts o = *opts;
     o.solver = GSLAM_SOLVER_PCG_FP64;
     gslam_solve_stats st{};
@@ -103,7 +101,6 @@ extern "C" int gslam_adjoint(gslam_graph *gh, const double *dl_dx, const gslam_s
     if (rc == GSLAM_OK) rc = gslam::launch_adjoint_grad(g, v, gw);
     std::vector<double> host(static_cast<size_t>(nb));
     if (rc == GSLAM_OK && nb > 0) rc = g.download(host.data(), gw, nb * 8);
-    (void)hipFree(b);
     if (stats) *stats = st;
     if (rc != GSLAM_OK) return rc;
     if (dl_dw && nb > 0) std::copy(host.begin(), host.end(), dl_dw);

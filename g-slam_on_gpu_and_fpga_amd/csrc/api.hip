@@ -31,7 +31,37 @@ DeviceGraph::~DeviceGraph()
     if (ev2) (void)hipEventDestroy(ev2);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     dense.release();
+    if (vecs) (void)hipFree(vecs);
+    if (csr_tmp) (void)hipFree(csr_tmp);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+int DeviceGraph::vecs_reserve()
+{
+    if (vecs) return GSLAM_OK;
+    const size_t count = static_cast<size_t>(4 * dim() + h.n_bet_e + 8);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&vecs), count * 8);
+    if (e != hipSuccess) {
+        vecs = nullptr;
+        set_error(std::string("hipMalloc(vector work space): ") + hipGetErrorString(e));
+        return GSLAM_ERR_NOMEM;
+    }
+    bytes += static_cast<int64_t>(count * 8);
+    return GSLAM_OK;
+}
+
+int DeviceGraph::csr_tmp_reserve()
+{
+    if (csr_tmp) return GSLAM_OK;
+    const size_t count = static_cast<size_t>(h.nnz > 0 ? h.nnz : 1);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&csr_tmp), count * 8);
+    if (e != hipSuccess) {
+        csr_tmp = nullptr;
+        set_error(std::string("hipMalloc(CSR staging): ") + hipGetErrorString(e));
+        return GSLAM_ERR_NOMEM;
+    }
+    bytes += static_cast<int64_t>(count * 8);
+    return GSLAM_OK;
 }
 
 int DeviceGraph::upload(void *dst, const void *src, int64_t nbytes)
@@ -185,6 +215,7 @@ void gslam_default_opts(gslam_solve_opts *o)
     o->check_every = 0;
     o->refine_mode = GSLAM_REFINE_IR;
     o->profile_every = 0;
+    o->mixed_handover = 0.0;
 }
 
 void gslam_default_desc(gslam_graph_desc *d)
@@ -305,13 +336,11 @@ int gslam_set_system_csr(gslam_graph *gh, const double *values, const double *xi
     if (!gh || !values || !xi) return GSLAM_ERR_ARG;
     DeviceGraph &g = gh->g;
     const int64_t nnz = g.h.nnz;
-    double *tmp = nullptr;
-    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&tmp), static_cast<size_t>(nnz > 0 ? nnz : 1) * 8));
-    int rc = g.upload(tmp, values, nnz * 8);
-    if (rc == GSLAM_OK) rc = gslam::launch_csr_to_bsr(g, tmp, g.bval);
+    GSLAM_TRY(g.csr_tmp_reserve());
+    int rc = g.upload(g.csr_tmp, values, nnz * 8);
+    if (rc == GSLAM_OK) rc = gslam::launch_csr_to_bsr(g, g.csr_tmp, g.bval);
     if (rc == GSLAM_OK) rc = g.upload(g.xi, xi, 3 * static_cast<int64_t>(g.h.n) * 8);
     (void)hipStreamSynchronize(g.stream);
-    (void)hipFree(tmp);
     if (rc != GSLAM_OK) return rc;
     g.assembled = true;
     g.factors_valid = false;
@@ -350,13 +379,9 @@ int gslam_get_system_csr(gslam_graph *gh, int32_t *row_ptr, int32_t *col_idx, do
     if (row_ptr) std::memcpy(row_ptr, h.csr_row_ptr.data(), h.csr_row_ptr.size() * 4);
     if (col_idx) std::memcpy(col_idx, h.csr_col_idx.data(), h.csr_col_idx.size() * 4);
     if (values) {
-        double *tmp = nullptr;
-        GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&tmp),
-                            static_cast<size_t>(h.nnz > 0 ? h.nnz : 1) * 8));
-        int rc = gslam::launch_bsr_to_csr(g, g.bval, tmp);
-        if (rc == GSLAM_OK) rc = g.download(values, tmp, h.nnz * 8);
-        (void)hipFree(tmp);
-        if (rc != GSLAM_OK) return rc;
+        GSLAM_TRY(g.csr_tmp_reserve());
+        GSLAM_TRY(gslam::launch_bsr_to_csr(g, g.bval, g.csr_tmp));
+        GSLAM_TRY(g.download(values, g.csr_tmp, h.nnz * 8));
     }
     if (xi) GSLAM_TRY(g.download(xi, g.xi, 3 * static_cast<int64_t>(h.n) * 8));
     return GSLAM_OK;
@@ -388,6 +413,10 @@ static int check_opts(const gslam_solve_opts *o)
     }
     if (!(o->tol >= 0.0) || o->max_iter < 0) {
         set_error("tol must be >= 0 and max_iter >= 0");
+        return GSLAM_ERR_ARG;
+    }
+    if (!(o->mixed_handover >= 0.0)) {
+        set_error("mixed_handover must be >= 0");
         return GSLAM_ERR_ARG;
     }
     if (o->solver == GSLAM_SOLVER_GJ_MIXED &&
@@ -441,15 +470,9 @@ int gslam_solve_rhs(gslam_graph *gh, const gslam_solve_opts *opts, const double 
         set_error("gslam_solve_rhs: call gslam_assemble first");
         return GSLAM_ERR_STATE;
     }
-    const int64_t dim = 3 * static_cast<int64_t>(g.h.n);
-    double *b = nullptr, *xd = nullptr;
-    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&b), static_cast<size_t>(dim) * 8));
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&xd), static_cast<size_t>(dim) * 8);
-    if (e != hipSuccess) {
-        (void)hipFree(b);
-        set_error(hipGetErrorString(e));
-        return GSLAM_ERR_NOMEM;
-    }
+    const int64_t dim = g.dim();
+    GSLAM_TRY(g.vecs_reserve());
+    double *b = g.vecs, *xd = g.vecs + dim;
     gslam_solve_stats s;
     int rc = g.upload(b, rhs, dim * 8);
     if (rc == GSLAM_OK) rc = solve_into(g, *opts, b, xd, s);
@@ -457,8 +480,6 @@ int gslam_solve_rhs(gslam_graph *gh, const gslam_solve_opts *opts, const double 
         int rc2 = g.download(x, xd, dim * 8);
         if (rc2 != GSLAM_OK) rc = rc2;
     }
-    (void)hipFree(b);
-    (void)hipFree(xd);
     if (stats) *stats = s;
     return rc;
 }

@@ -129,6 +129,16 @@ struct DeviceGraph {
     int32_t *xoth = nullptr, *xpos = nullptr;
     float *frec = nullptr;
 
+    // Vector work space, allocated on first use and kept: 4 x 3N + n_between
+    // doubles (caller right-hand side and solution of gslam_solve_rhs and
+    // gslam_adjoint, the mixed solver's FP64 correction r and d, adjoint
+    // edge gradients), and an nnz-double staging buffer for the CSR copies.
+    double *vecs = nullptr;
+    double *csr_tmp = nullptr;
+    int vecs_reserve();
+    int csr_tmp_reserve();
+    int64_t dim() const { return 3 * static_cast<int64_t>(h.n); }
+
     DenseWork dense;             // GJ_MIXED work space (allocated on first use)
     DenseWork64 dense64;         // GJ_BLOCKED64 work space (allocated on first use)
     std::vector<hipEvent_t> prof_ev;  // in-solve kernel timing (created on first use)

@@ -449,8 +449,98 @@ mixed_finalize_kernel(const double *__restrict__ pr_cur, int32_t np_cur, int32_t
     }
 }
 
+// r = b - Omega x (FP64 BSR), for the FP64 correction after the FP32 phase.
+__global__ void __launch_bounds__(kThreads)
+resid64_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+               const double *__restrict__ bval, const double *__restrict__ b,
+               const double *__restrict__ x, double *__restrict__ r, int32_t n)
+{
+    constexpr int G = kLanesMixed, kRowsPerTile = kThreads / G;
+    const int lane = threadIdx.x % G;
+    for (int64_t row0 = static_cast<int64_t>(blockIdx.x) * kRowsPerTile; row0 < n;
+         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
+        const int64_t row = row0 + threadIdx.x / G;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < n) {
+            const int32_t b1 = brow_ptr[row + 1];
+            for (int32_t bb = brow_ptr[row] + lane; bb < b1; bb += G) {
+                const int64_t j = 3 * static_cast<int64_t>(bcol[bb]);
+                const double *v = bval + 9 * static_cast<int64_t>(bb);
+                y0 += v[0] * x[j] + v[1] * x[j + 1] + v[2] * x[j + 2];
+                y1 += v[3] * x[j] + v[4] * x[j + 1] + v[5] * x[j + 2];
+                y2 += v[6] * x[j] + v[7] * x[j + 1] + v[8] * x[j + 2];
+            }
+        }
+        y0 = group_sum<G>(y0);
+        y1 = group_sum<G>(y1);
+        y2 = group_sum<G>(y2);
+        if (row < n && lane == 0) {
+            const int64_t o = 3 * row;
+            r[o] = b[o] - y0;
+            r[o + 1] = b[o + 1] - y1;
+            r[o + 2] = b[o + 2] - y2;
+        }
+    }
+}
+
+// x += d
+__global__ void __launch_bounds__(kThreads)
+add64_kernel(double *__restrict__ x, const double *__restrict__ d, int64_t m)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < m;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        x[i] += d[i];
+}
+
+static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                       gslam_solve_stats &s);
+
+// FP32 phase, then (mixed_handover > 0) FP64 iterative refinement: once the
+// FP64 true residual of the FP32 phase is below mixed_handover ||b||, FP64
+// PCG solves Omega d = b - Omega x to the remaining tolerance and x += d.
+// The FP32 recurrence cannot resolve the modes of the block-Jacobi
+// preconditioned operator whose eigenvalues lie below ~eps32 of the largest
+// (kappa up to ~1e13 on long grid-city chains), so the FP64 correction is
+// what certifies tol on such graphs.
 int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
                     gslam_solve_stats &s)
+{
+    const double handover = o.mixed_handover;
+    gslam_solve_opts o1 = o;
+    if (handover > o.tol) o1.tol = handover;
+    GSLAM_TRY(mixed_phase(g, o1, b, x_out, s));
+    s.iters_fp64 = 0;
+    if (!(handover > 0.0) || s.true_rel_res < o.tol || (s.flags & 1)) return GSLAM_OK;
+    GSLAM_TRY(g.vecs_reserve());
+    const int64_t dim = g.dim();
+    double *r2 = g.vecs + 2 * dim, *d = g.vecs + 3 * dim;
+    const int32_t n = g.h.n;
+    const unsigned int gr = grid_for(n, kLanesMixed);
+    hipLaunchKernelGGL(resid64_kernel, dim3(gr), dim3(kThreads), 0, g.stream, g.brow_ptr, g.bcol,
+                       g.bval, b, x_out, r2, n);
+    GSLAM_HIP(hipGetLastError());
+    gslam_solve_opts o2 = o;
+    o2.solver = GSLAM_SOLVER_PCG_FP64;
+    o2.tol = s.true_rel_res > 0.0 ? o.tol / s.true_rel_res : o.tol;
+    o2.max_iter = std::max(0, o.max_iter - s.iters);
+    o2.profile_every = 0;
+    gslam_solve_stats s2{};
+    GSLAM_TRY(pcg_solve_fp64(g, o2, r2, d, s2));
+    hipLaunchKernelGGL(add64_kernel, dim3(grid_for(dim, 1)), dim3(kThreads), 0, g.stream, x_out, d,
+                       dim);
+    GSLAM_HIP(hipGetLastError());
+    s.iters += s2.iters;
+    s.iters_fp64 = s2.iters;
+    s.ms += s2.ms;
+    s.flags |= s2.flags;
+    GSLAM_TRY(true_rel_residual(g, b, x_out, &s.true_rel_res));
+    s.converged = s2.converged;  // the FP64 recurrence's test, as for PCG_FP64
+    s.rel_res = s.true_rel_res;
+    return GSLAM_OK;
+}
+
+static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
+                       gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
     const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);

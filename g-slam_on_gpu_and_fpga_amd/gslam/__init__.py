@@ -58,14 +58,14 @@ class _Opts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("refine_sweeps", C.c_int32), ("refresh_every", C.c_int32),
                 ("check_every", C.c_int32), ("refine_mode", C.c_int32),
-                ("profile_every", C.c_int32)]
+                ("profile_every", C.c_int32), ("mixed_handover", C.c_double)]
 
 
 class _Stats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("reliable_updates", C.c_int32),
                 ("flags", C.c_int32), ("rel_res", C.c_double), ("true_rel_res", C.c_double),
                 ("ms", C.c_double), ("ms_factor", C.c_double), ("spmv_us", C.c_double),
-                ("update_us", C.c_double)]
+                ("update_us", C.c_double), ("iters_fp64", C.c_int32)]
 
 
 _LIB = None
@@ -209,7 +209,7 @@ def read_log(path: str) -> dict:
 
 
 def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
-              check_every=0, refine_mode="ir", profile_every=0) -> _Opts:
+              check_every=0, refine_mode="ir", profile_every=0, mixed_handover=0.0) -> _Opts:
     o = _Opts()
     lib().gslam_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -220,13 +220,15 @@ def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_e
     o.check_every = check_every
     o.refine_mode = REFINE[refine_mode] if isinstance(refine_mode, str) else int(refine_mode)
     o.profile_every = profile_every
+    o.mixed_handover = mixed_handover
     return o
 
 
 def _stats(s: _Stats) -> dict:
     return dict(iters=s.iters, converged=bool(s.converged), reliable_updates=s.reliable_updates,
                 flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms,
-                ms_factor=s.ms_factor, spmv_us=s.spmv_us, update_us=s.update_us)
+                ms_factor=s.ms_factor, spmv_us=s.spmv_us, update_us=s.update_us,
+                iters_fp64=s.iters_fp64)
 
 
 class Graph:

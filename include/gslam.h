@@ -117,6 +117,12 @@ typedef struct {
     int32_t profile_every;      /* PCG_FP64: bracket the SpMV and update
                                    launches of every k-th iteration (at most
                                    128) with HIP events; 0 = off             */
+    double mixed_handover;      /* PCG_MIXED: FP64 true relative residual at
+                                   which the FP32 iterations stop and FP64
+                                   PCG solves the remaining correction
+                                   Omega d = xi - Omega dx to tol (FP64
+                                   iterative refinement); 0 = FP32 iterations
+                                   with FP64 refreshes down to tol           */
 } gslam_solve_opts;
 
 /* GJ_MIXED refinement on the true residual xi - Omega dx (FP64), with the
@@ -142,6 +148,9 @@ typedef struct {
     double ms_factor;           /* GJ: device time of the dense inversion     */
     double spmv_us, update_us;  /* PCG_FP64 with profile_every: mean sampled
                                    per-launch durations inside the solve (us) */
+    int32_t iters_fp64;         /* PCG_MIXED with mixed_handover: FP64 PCG
+                                   iterations of the correction (included in
+                                   iters)                                    */
 } gslam_solve_stats;
 
 /* Fills opts with the reference defaults (PCG FP64, tol 1e-6, 500 iterations:

@@ -98,29 +98,40 @@ def test_c4a_pcg64_grid100k(grid100k):
 
 
 def test_c4a_mixed_grid100k(grid100k):
+    """FP32 iterations with FP64 true-residual refreshes stall at ~4e-7 here
+    (the block-Jacobi preconditioned operator has kappa ~ 1e13, far beyond
+    1/eps32); with the FP64 correction (mixed_handover) the mixed solver meets
+    the FP64 solver's accuracy."""
     g, x_ref = grid100k
     st = g.solve("mixed", 1e-12, 1500000)
+    e0 = rel(g.dx(), x_ref)
+    assert st["converged"] and st["true_rel_res"] < 1e-12 and st["iters_fp64"] == 0
+    st = g.solve("mixed", 3e-13, 1500000, mixed_handover=1e-4)
     dx = g.dx()
     e = rel(dx, x_ref)
     em, er = _maxerr(dx, x_ref)
-    print(f"C4a mixed N=100k tol 1e-12: {st['iters']} its ({st['reliable_updates']} refreshes), "
+    print(f"C4a mixed N=100k: FP32 only (tol 1e-12) rel dx err {e0:.3e}; with the FP64 "
+          f"correction below 1e-4 (tol 3e-13): {st['iters']} its ({st['iters_fp64']} FP64), "
           f"rel dx err {e:.3e}, max pose err {em:.2e} m {er:.2e} rad, true rel res "
           f"{st['true_rel_res']:.2e}")
-    assert st["converged"] and st["true_rel_res"] < 1e-12
-    assert e <= 1e-6, e
+    assert 1e-7 <= e0 <= 1e-6, e0                     # measured 3.6e-7: the FP32 floor
+    assert st["converged"] and 0 < st["iters_fp64"] < st["iters"]
+    assert e <= 1e-8, e                               # measured 8.1e-9
 
 
-@pytest.mark.parametrize("solver,tol", [("pcg64", 1e-12), ("mixed", 1e-12)])
-def test_c4b_park7120(solver, tol):
+@pytest.mark.parametrize("solver,tol,kw", [("pcg64", 1e-12, {}), ("mixed", 1e-12, {}),
+                                           ("mixed", 1e-10, dict(mixed_handover=1e-2))])
+def test_c4b_park7120(solver, tol, kw):
     gc = G.park(7120, seed=0, route="laps")
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 1.0)) as g:
         g.assemble()
         x_ref = refined_direct(*g.system_csr())
-        st = g.solve(solver, tol, 400000)
+        st = g.solve(solver, tol, 400000, **kw)
         dx = g.dx()
     e = rel(dx, x_ref)
     em, er = _maxerr(dx, x_ref)
-    print(f"C4b {solver} park N=7120: {st['iters']} its, rel dx err {e:.3e}, max pose err "
-          f"{em:.2e} m {er:.2e} rad, {st['ms']:.1f} ms")
+    print(f"C4b {solver} {kw} park N=7120: {st['iters']} its ({st['iters_fp64']} FP64), rel dx "
+          f"err {e:.3e}, max pose err {em:.2e} m {er:.2e} rad, {st['ms']:.1f} ms")
     assert st["converged"]
-    assert e <= 1e-10, e                            # measured 1.9e-13 (FP64), 2.3e-12 (mixed)
+    # measured 1.9e-13 (FP64), 2.3e-12 (mixed), 9.6e-12 (mixed, FP64 correction)
+    assert e <= 1e-10, e
