@@ -16,6 +16,7 @@
 #include "device_common.h"
 #include "gslam_internal.h"
 
+
 namespace gslam {
 
 constexpr int kGjTile = 16;  // 16 x 16 threads per workgroup, one element each
@@ -59,6 +60,8 @@ gj_column_kernel(const double *__restrict__ m, double *__restrict__ col, int64_t
     const int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (j < n) col[j] = m[j * n + c];
 }
+
+__global__ void gj_flag_clear_kernel(int32_t *flag) { *flag = 0; }
 
 __global__ void __launch_bounds__(kThreads)
 gj_scale_kernel(double *__restrict__ m, const double *__restrict__ diag, int64_t n,
@@ -119,7 +122,7 @@ int gj_invert_fp64_device(hipStream_t s, double *m, int64_t n, double *work, int
                           int32_t *singular_out)
 {
     double *col0 = work, *col1 = work + n, *diag = work + 2 * n;
-    GSLAM_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(gj_flag_clear_kernel, dim3(1), dim3(1), 0, s, flag);
     hipLaunchKernelGGL(gj_column_kernel, dim3(nb1(n)), dim3(kThreads), 0, s, m, col0, n, int64_t(0));
     const unsigned int gt = static_cast<unsigned int>((n + kGjTile - 1) / kGjTile);
     for (int64_t i = 0; i < n; ++i) {
@@ -156,16 +159,12 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
         set_error("dense solve limited to 3N <= 60000 (28.8 GB FP64 matrix)");
         return GSLAM_ERR_TOO_LARGE;
     }
-    double *m = nullptr, *work = nullptr;
-    int32_t *flag = nullptr;
-    GSLAM_HIP(hipMalloc(reinterpret_cast<void **>(&m), static_cast<size_t>(n * n) * 8));
-    if (hipMalloc(reinterpret_cast<void **>(&work), static_cast<size_t>(3 * n + 8) * 8) != hipSuccess ||
-        hipMalloc(reinterpret_cast<void **>(&flag), 64) != hipSuccess) {
-        (void)hipFree(m);
-        (void)hipFree(work);
-        set_error("dense solve: out of device memory");
-        return GSLAM_ERR_NOMEM;
-    }
+    // the GJ_BLOCKED64 work space, kept on the handle between solves: its
+    // padded matrix (np >= n) holds the n x n matrix, its column panel
+    // (128 np >= 3n + 8 doubles) the pivot columns and the diagonal
+    GSLAM_TRY(g.dense64.reserve(n));
+    double *m = g.dense64.a, *work = g.dense64.ct;
+    int32_t *flag = g.dense64.flag;
     hipStream_t s = g.stream;
     (void)hipEventRecord(g.ev0, s);
     hipLaunchKernelGGL(fill_kernel, dim3(nb1(n * n)), dim3(kThreads), 0, s, m, n * n, 0.0);
@@ -181,9 +180,6 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, g.ev0, g.ev1);
-    (void)hipFree(m);
-    (void)hipFree(work);
-    (void)hipFree(flag);
     if (rc != GSLAM_OK) return rc;
     st.ms = ms;
     st.ms_factor = ms;
