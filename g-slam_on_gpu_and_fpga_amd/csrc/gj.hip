@@ -16,6 +16,9 @@
 #include "device_common.h"
 #include "gslam_internal.h"
 
+#include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace gslam {
 
@@ -62,6 +65,14 @@ gj_column_kernel(const double *__restrict__ m, double *__restrict__ col, int64_t
 }
 
 __global__ void gj_flag_clear_kernel(int32_t *flag) { *flag = 0; }
+
+__global__ void __launch_bounds__(kThreads)
+copy64_kernel(const double *__restrict__ src, double *__restrict__ dst, int64_t m)
+{
+    for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < m;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        dst[t] = src[t];
+}
 
 __global__ void __launch_bounds__(kThreads)
 gj_scale_kernel(double *__restrict__ m, const double *__restrict__ diag, int64_t n,
@@ -194,7 +205,73 @@ int dense_solve(DeviceGraph &g, const gslam_solve_opts &o, const double *b, doub
     return GSLAM_OK;
 }
 
+// ---- allocation-free entry point (gslam_gj_invert_device) ----
+// Scratch of the in-place FP64 Gauss-Jordan -- the two pivot-column buffers,
+// the diagonal (3n doubles) and the zero-pivot flag -- in a file-scope
+// device array, so that the entry point calls no allocator.  Every GPU the
+// library runs on holds its own copy until the process ends.  Host code
+// must not pass the array's name to a kernel (it is a host-side stand-in):
+// the device address comes from hipGetSymbolAddress, once per device.
+constexpr int32_t kNoAllocMaxN = 16384;
+constexpr int64_t kScratchDoubles = 3 * static_cast<int64_t>(kNoAllocMaxN) + 8;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_address(double **out)
+{
+    int dev = 0;
+    GSLAM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("gslam_gj_invert_device: device ordinal beyond 64");
+        return GSLAM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("gslam_gj_invert_device: hipGetSymbolAddress(scratch): ") +
+                      hipGetErrorString(e));
+            return GSLAM_ERR_HIP;
+        }
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return GSLAM_OK;
+}
+
 }  // namespace gslam
+
+extern "C" int gslam_gj_invert_device(const double *a, double *inv, int32_t n)
+{
+    using namespace gslam;
+    if (!a || !inv || n <= 0) return GSLAM_ERR_ARG;
+    if (n > kNoAllocMaxN) {
+        set_error("gslam_gj_invert_device: n = " + std::to_string(n) + " beyond the " +
+                  std::to_string(kNoAllocMaxN) + " its static scratch holds (3n + 8 doubles); "
+                  "use gslam_gj_invert");
+        return GSLAM_ERR_TOO_LARGE;
+    }
+    double *work = nullptr;
+    GSLAM_TRY(scratch_address(&work));
+    int32_t *flag = reinterpret_cast<int32_t *>(work + 3 * static_cast<int64_t>(n));
+    const int64_t nn = static_cast<int64_t>(n) * n;
+    hipStream_t s = nullptr;  // the null stream: creating one would allocate
+    if (a != inv) {
+        const unsigned int gr = static_cast<unsigned int>(std::min<int64_t>(4096, nb1(nn)));
+        hipLaunchKernelGGL(copy64_kernel, dim3(gr), dim3(kThreads), 0, s, a, inv, nn);
+    }
+    int32_t singular = 0;
+    GSLAM_TRY(gj_invert_fp64_device(s, inv, n, work, flag, &singular));
+    if (singular) {
+        set_error("Gauss-Jordan: zero pivot (the inverse is the zero matrix)");
+        return GSLAM_ERR_SINGULAR;
+    }
+    return GSLAM_OK;
+}
 
 extern "C" int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver)
 {

@@ -77,7 +77,7 @@ EXPORTS = [
     "gslam_set_log_weights", "gslam_assemble", "gslam_get_system_csr", "gslam_get_system_bsr",
     "gslam_solve", "gslam_get_dx", "gslam_solve_rhs", "gslam_apply_update", "gslam_gn_step",
     "gslam_gj_invert", "gslam_last_error", "gslam_set_system_csr", "gslam_time_kernel",
-    "gslam_adjoint", "gslam_obs_cost",
+    "gslam_adjoint", "gslam_obs_cost", "gslam_gj_invert_device",
     "device_graph_init", "device_graph_update", "device_graph_get_row_ptr",
     "device_graph_get_col_idx", "device_build_diag_inv", "device_pcg_solve",
     "device_graph_free", "inv_cuda",

@@ -233,6 +233,20 @@ int gslam_gn_step(gslam_graph *g, const gslam_solve_opts *opts,
  * (inv is then all zeros, as in the reference). */
 int gslam_gj_invert(const double *a, double *inv, int32_t n, int32_t solver);
 
+/* FP64 Gauss-Jordan inverse of a device-resident n x n matrix into the
+ * caller's device buffer inv (a == inv inverts in place), allocating
+ * nothing: the same kernels as gslam_gj_invert(..., GSLAM_SOLVER_GJ_FP64)
+ * with their scratch (3n + 8 doubles: pivot columns, diagonal, zero-pivot
+ * flag) in a static device array of 3 x 16384 + 8 doubles per GPU.  No
+ * counterpart in the reference (its inv_cuda, src/gpu_gj_fp64/inv.h:9,
+ * allocates per call); for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  n <= 16384, else GSLAM_ERR_TOO_LARGE.  Runs on the null
+ * stream of the current device and returns when the inverse is complete;
+ * GSLAM_ERR_SINGULAR on a zero pivot (inv is then all zeros).  Calls that
+ * overlap on one GPU share its scratch and race. */
+int gslam_gj_invert_device(const double *a, double *inv, int32_t n);
+
 /* Adjoint of the converged poses x* of a between-edge graph (no reference
  * counterpart; the reference tunes lambda by a manual grid search,
  * README.md:137-163).  With f = 1/2 sum_e e_e^T L_e e_e (+ anchor prior),
