@@ -282,14 +282,14 @@ class Graph:
         self.info = {k: getattr(info, k) for k, _ in _Info._fields_}
 
     @classmethod
-    def from_log(cls, path: str, lam: float = 0.1, pair_order: int = 0, wrap: int = 0,
-                 data: dict | None = None):
+    def from_log(cls, path: str | None = None, lam: float = 0.1, pair_order: int = 0,
+                 wrap: int = 0, data: dict | None = None, anchor_prior: int = 0):
         g = read_log(path) if data is None else data
         return cls(g["poses"], motion=True, u_nu=g["u_nu"], u_omega=g["u_omega"],
                    delta=g["delta"], lam=lam,
                    obs=dict(z_step=g["z_step"], z_lm=g["z_lm"], z_range=g["z_range"],
                             z_bearing=g["z_bearing"]),
-                   pair_order=pair_order, wrap=wrap)
+                   pair_order=pair_order, wrap=wrap, anchor_prior=anchor_prior)
 
     # -------------------------------------------------------------- lifecycle
     def close(self):
@@ -372,8 +372,11 @@ class Graph:
 
     def adjoint(self, dl_dx, tol=1e-12, max_iter=200000):
         """dL/dw_e (between edges) and dL/dlambda at the current (converged)
-        poses for the caller's dL/dx*: one FP64 PCG on the exact Hessian and an
-        edge-parallel gradient kernel."""
+        poses for the caller's dL/dx*.  Between-edge graphs: one FP64 PCG on
+        the exact Hessian and an edge-parallel gradient kernel.  Graphs of the
+        reference's landmark-pair and motion edges: dL/dlambda by a tangent
+        solve (GMRES preconditioned by the FP64 PCG; tol is then its relative
+        residual), no per-edge weights (dl_dw is empty)."""
         o = make_opts("pcg64", tol, max_iter)
         g = _f64(dl_dx, 3 * self.n)
         dw = np.zeros(self.info["n_between_edges"])

@@ -95,3 +95,45 @@ def tune(graph: Graph, init_poses, truth, steps: int = 50, lr: float = 0.05,
         opt.step()
         hist.append(float(loss.detach()))
     return log_w.detach(), float(log_lam.detach().exp()), hist
+
+
+def drift_loss(poses) -> tuple:
+    """The README's lambda-search criterion (README.md:137-163): squared
+    distance between the first and the last pose position, and its gradient
+    with respect to the poses (3N)."""
+    p = np.asarray(poses).reshape(-1, 3)
+    d = p[-1, :2] - p[0, :2]
+    g = np.zeros_like(p)
+    g[-1, :2] = 2.0 * d
+    g[0, :2] = -2.0 * d
+    return float(d @ d), g.ravel()
+
+
+def tune_lambda_drift(graph: Graph, init_poses, steps: int = 10, lr: float = 0.1,
+                      lam0: float = 1.0, converge_kw: dict | None = None, adjoint_tol=1e-10):
+    """Adam on log lambda of the converged poses' squared drift -- the
+    gradient replacement of the README's coarse-then-fine lambda grid search
+    (README.md:137-163).  Works on the reference's landmark-pair / motion edge
+    model (graph created with anchor_prior=1): each step runs Gauss-Newton to
+    convergence from init_poses, then gslam_adjoint's tangent solve for
+    dL/dlambda.  Returns (lambda, history of (lambda, L, dL/dlambda))."""
+    kw = dict(solver="pcg64", tol=1e-13, max_iter=400000, diff_tol=1e-12, max_rounds=3000,
+              patience=30)
+    kw.update(converge_kw or {})
+    log_lam = float(np.log(lam0))
+    m = v = 0.0
+    b1, b2, eps = 0.9, 0.999, 1e-12
+    hist = []
+    for t in range(1, steps + 1):
+        lam = float(np.exp(log_lam))
+        graph.set_lambda(lam)
+        graph.set_poses(init_poses)
+        graph.converge(**kw)
+        L, g = drift_loss(graph.get_poses())
+        _, dlam, _ = graph.adjoint(g, tol=adjoint_tol)
+        hist.append((lam, L, dlam))
+        grad = dlam * lam                      # dL / d log lambda
+        m = b1 * m + (1 - b1) * grad
+        v = b2 * v + (1 - b2) * grad * grad
+        log_lam -= lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    return float(np.exp(log_lam)), hist

@@ -243,3 +243,94 @@ def write_pose_graph(g: dict, log_path: str, between_path: str) -> None:
         for i, j, a, b in zip(g["i"], g["j"], m, inf):
             f.write(f"e {int(i)} {int(j)} {a[0]:.17g} {a[1]:.17g} {a[2]:.17g} "
                     f"{b[0]:.17g} {b[1]:.17g} {b[2]:.17g}\n")
+
+
+def _velocity_step(x, y, th, nu, om, delta=1.0):
+    """The velocity motion model the reference's motion edge linearises
+    (state_transition, src/cpu_pcg_fp64/GSLAM.c:373-401)."""
+    if abs(om) < 1e-10:
+        return x + nu * np.cos(th) * delta, y + nu * np.sin(th) * delta, th
+    return (x + nu / om * (np.sin(th + om * delta) - np.sin(th)),
+            y + nu / om * (-np.cos(th + om * delta) + np.cos(th)), th + om * delta)
+
+
+def landmark_world(n_poses: int, seed: int = 0, block: int = 25, odo_noise: float = 1.0,
+                   obs_noise=(0.05, 0.01), sensor_range: float = 8.5, density: float = 3e-3,
+                   max_obs_per_landmark: int = 10, strip: float = 20.0) -> dict:
+    """A landmark world in the reference's .log.txt model (u / x / z records),
+    after data/generate_citytrees.py:59-204: a robot drives laps of a closed
+    block x block route of 1 m steps with 90 degree turns (so the last pose
+    returns near the first, which makes the README's end-to-start drift a
+    meaningful loss), trees are scattered along the route and observed by
+    range and bearing within sensor_range (at most max_obs_per_landmark
+    times each).  Unlike the reference generator, the controls written to the
+    log are the noisy odometry (motion noise of the reference's motion edge,
+    src/cpu_pcg_fp64/MotionEdge.c:23-26, scaled by odo_noise), the initial
+    poses are dead-reckoned from them, and the observations carry Gaussian
+    range / bearing noise obs_noise: with consistent data every residual
+    vanishes at the solution and lambda has no effect on it.  Returns the
+    fields of gslam.read_log plus the true poses ('truth')."""
+    rng = np.random.default_rng(seed)
+    us = [(0.0, 0.0)]
+    k = 1
+    while len(us) < n_poses:
+        for _ in range(block - 1):
+            us.append((1.0, 0.0))
+        us.append((1.0, np.pi / 2))
+        k += block
+    us = us[:n_poses]
+    gt = [(0.0, 0.0, 0.0)]
+    for nu, om in us[1:]:
+        gt.append(_velocity_step(*gt[-1], nu, om))
+    gt = np.array(gt)
+    odo = [(0.0, 0.0)]
+    for nu, om in us[1:]:
+        sn = np.sqrt(0.19 ** 2 * abs(nu) + 0.001 ** 2 * abs(om) + 1e-9) * odo_noise
+        so = np.sqrt(0.13 ** 2 * abs(nu) + 0.2 ** 2 * abs(om) + 1e-9) * odo_noise
+        odo.append((nu + rng.normal(0.0, sn), om + rng.normal(0.0, so)))
+    init = [(0.0, 0.0, 0.0)]
+    for nu, om in odo[1:]:
+        init.append(_velocity_step(*init[-1], nu, om))
+    init = np.array(init)
+    init[:, 2] = _wrap(init[:, 2])
+    lo, hi = gt[:, :2].min(0) - strip, gt[:, :2].max(0) + strip
+    n_lm = max(10, int(np.prod(hi - lo) * density))
+    lms = []
+    while len(lms) < n_lm:
+        m = rng.uniform(lo, hi)
+        if (np.hypot(*(gt[:, :2] - m).T) <= sensor_range).any():
+            lms.append(m)
+    z, cnt = [], np.zeros(len(lms), np.int64)
+    for s, (px, py, pt) in enumerate(gt):
+        for li, (mx, my) in enumerate(lms):
+            if cnt[li] >= max_obs_per_landmark:
+                continue
+            d = np.hypot(mx - px, my - py)
+            if d > sensor_range:
+                continue
+            b = _wrap(np.arctan2(my - py, mx - px) - pt)
+            z.append((s, li + 6, d + rng.normal(0.0, obs_noise[0]), b + rng.normal(0.0, obs_noise[1])))
+            cnt[li] += 1
+    z = np.array(z).reshape(-1, 4)
+    return dict(delta=1.0, u_step=np.arange(n_poses, dtype=np.int32),
+                u_nu=np.array([u[0] for u in odo]), u_omega=np.array([u[1] for u in odo]),
+                x_step=np.arange(n_poses, dtype=np.int32), poses=init.ravel().copy(),
+                z_step=z[:, 0].astype(np.int32), z_lm=z[:, 1].astype(np.int32),
+                z_range=z[:, 2].copy(), z_bearing=z[:, 3].copy(), n_poses=n_poses, truth=gt)
+
+
+def write_log(g: dict, path: str) -> None:
+    """Writes a landmark world as a reference .log.txt ("delta", then per step
+    its u, x and z records, the layout of data/generate_citytrees.py:178-196),
+    values as %.17g."""
+    zs = {}
+    for s, l, r, b in zip(g["z_step"], g["z_lm"], g["z_range"], g["z_bearing"]):
+        zs.setdefault(int(s), []).append((int(l), r, b))
+    p = np.asarray(g["poses"]).reshape(-1, 3)
+    with open(path, "w") as f:
+        f.write(f"delta {g['delta']:.17g}\n")
+        for s in range(g["n_poses"]):
+            f.write(f"u {s} {g['u_nu'][s]:.17g} {g['u_omega'][s]:.17g}\n")
+            f.write(f"x {s} {p[s, 0]:.17g} {p[s, 1]:.17g} {p[s, 2]:.17g}\n")
+            for l, r, b in zs.get(s, []):
+                f.write(f"z {s} {l} {r:.17g} {b:.17g}\n")

@@ -258,7 +258,21 @@ int gslam_gj_invert_device(const double *a, double *inv, int32_t n);
  * edge-parallel kernel forms
  *   dL/dw_e    = -(J_e v)^T L_e e_e            (n_between doubles, dl_dw)
  *   dL/dlambda = sum_e dL/dw_e / lambda          (dl_dlambda)
- * Requires desc.anchor_prior = 1, converged poses, and between edges only.
+ * Requires desc.anchor_prior = 1 and converged poses.
+ *
+ * Graphs of the reference's landmark-pair and motion edges (no between
+ * edges; dl_dw is not written): xi is not the gradient of a cost there (the
+ * edge information depends on the poses), so dL/dlambda comes from the
+ * tangent u = dx* / dlambda: K u = d xi / d lambda with K = -d xi / dx at
+ * x*, applied by central differences of the assembled xi, solved by
+ * right-preconditioned GMRES(40) with the FP64 PCG on Omega(x*) as the
+ * preconditioner (opts->tol: GMRES relative residual and PCG tolerance;
+ * at most min(opts->max_iter, 400) GMRES iterations, one restart); the
+ * result is accepted once the residual through a fresh central-difference
+ * product (stats->true_rel_res, ~1e-5 from their rounding) is <= 1e-4;
+ * stats->iters counts GMRES iterations, stats->iters_fp64 PCG iterations.  lambda scales the
+ * motion edges only (MotionEdge.c:113-175), so d xi / d lambda =
+ * (xi(lambda) - xi(0)) / lambda.
  * Leaves the handle unassembled (the next solve assembles again). */
 int gslam_adjoint(gslam_graph *g, const double *dl_dx, const gslam_solve_opts *opts,
                   double *dl_dw, double *dl_dlambda, gslam_solve_stats *stats);
