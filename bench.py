@@ -79,6 +79,14 @@ def update_bytes(n_poses):
     return n_poses * (4 * 24 + 72 + 3 * 24)
 
 
+def mixed_spmv_bytes(n_poses, n_inc):
+    """Algorithmic bytes of one mixed (FP32 factorized) SpMV launch: per
+    incidence entry (two per edge) 8 FP32 factors, the other pose's index and
+    the edge entry (40 B), per pose its row pointer and p_old, z read and p_new,
+    q written (FP32: 4 x 12 B)."""
+    return n_inc * 40 + n_poses * 48
+
+
 def measured_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -537,6 +545,16 @@ def profile_kernel(args, gslam, G):
         log(f"stream: 2 GiB copy {us:.0f} us, {gbs:.0f} GB/s read+write = "
             f"{100 * gbs / HBM_PEAK_GBS:.1f}% of the 8 TB/s HBM peak")
         return
+    if args.profile_kernel == "spmv_mixed":
+        gc = G.grid_city(args.n, seed=args.seed)
+        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+            g.assemble()
+            us = g.time_kernel("spmv_mixed", 20)
+            byts = mixed_spmv_bytes(g.info["n_poses"], 2 * g.info["n_edges"])
+            gbs = byts / (us * 1e-6) / 1e9
+            log(f"spmv_mixed N={args.n}: {us:.2f} us/launch (23 launches), {byts / 1e6:.1f} MB "
+                f"algorithmic, {gbs:.0f} GB/s = {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
+        return
     if args.profile_kernel in ("spmv64", "update64"):
         k = args.profile_kernel
         gc = G.grid_city(args.n, seed=args.seed)
@@ -596,7 +614,8 @@ def main():
     ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
     ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
-    ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "gj_trailing", "copy", "stream"],
+    ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "spmv_mixed", "gj_trailing",
+                                                 "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "
                          "passes; 'copy' is a 64 MiB device copy for calibrating the counters, "
                          "'stream' times a 2 GiB device copy: the achievable HBM bandwidth)")
