@@ -146,11 +146,16 @@ def test_grid1600_cg_refinement_beyond_fp32_conditioning():
         us = g.time_kernel("gj_trailing", 5)
         assert us > 0
     print(f"grid1600: 3 IR sweeps rel err {e_ir:.3e}, CG {cg['iters']} its rel err {e_cg:.3e}")
-    # kappa(scaled Omega) * eps32 ~ 26: the FP32 inverse has no correct digit
-    # in the smoothest modes and 3 plain sweeps stay far from FP64 accuracy
-    # (0.15 measured, 7e-3 in the numpy emulation -- FP32 summation order
-    # decides); CG on the same operator converges.
-    assert e_ir < 1.0, e_ir
+    # kappa(scaled Omega) * eps32 = 26: the FP32 inverse has no correct digit
+    # in the smoothest modes and 3 plain sweeps stay far from FP64 accuracy.
+    # Measured 0.10 here.  On the CPU, the same Jacobi-scaled matrix inverted
+    # by the reference's unpivoted FP32 Gauss-Jordan (inv_float,
+    # src/cpu_gj_mixed/MatrixOps.c:174-212, as restated by the oracle) gives
+    # 0.27 after 3 sweeps, and LAPACK's LU-based FP32 inverse 2.5e-3 ... 3.0e-2
+    # depending on one-ulp changes to the matrix (the sweeps contract by only
+    # rho(I - M S) = 0.5-0.75), so the blocked GPU inverse sits between the
+    # two algorithms; CG on the same operator converges.
+    assert 0.01 < e_ir < 0.27, e_ir
     assert cg["converged"] and e_cg <= 1e-8, (cg, e_cg)
 
 
