@@ -38,7 +38,8 @@ struct PcgStatus {
     int32_t iters;       // iterations completed
     int32_t flags;       // bit0: p.q == 0 (MatrixOps.c:357)
     int32_t refreshes;   // mixed: FP64 true-residual refreshes
-    int32_t pad;
+    int32_t x_iter;      // FP64 PCG: k when x_k = x_(k-1) + alpha_(k-1) p_(k-1) is
+                         // still to be applied (by SpMV(k), or after the loop)
     double rnorm;        // ||r|| of the last test
     double bnorm;        // ||b||
     double tol;          // tol_rel * ||b||

@@ -12,8 +12,11 @@
 //                  p = z + beta p_old on the fly for every gathered column
 //                  (double-buffered p), q = A p, and leaves p.q partials;
 //   B(k) "update": sums the p.q partials, alpha = rz/pq (rz as A(k) reduced
-//                  it), updates x, r, z for its own 3x3 block rows and leaves
-//                  r.r / r.z partials.
+//                  it), updates r, z for its own 3x3 block rows, leaves
+//                  r.r / r.z partials and alpha; x_(k+1) = x_k + alpha_k p_k
+//                  is applied by A(k+1), which holds p_k for its output rows
+//                  anyway (or after the loop), the same arithmetic one launch
+//                  later.
 // Kernel boundaries order the partials; a grid-wide reduction through one
 // atomic counter costs ~15-35 us per iteration on MI355X at 1-2k
 // workgroups, the redundant re-read of <= 512 partials ~1 us.  Only
@@ -89,6 +92,7 @@ pcg_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ b
         status->iters = 0;
         status->flags = 0;
         status->refreshes = 0;
+        status->x_iter = -1;
         status->rnorm = 0.0;
         status->bnorm = 0.0;
         status->tol = 0.0;
@@ -136,7 +140,7 @@ tile_pack_kernel(const int4 *__restrict__ tile_info, const double *__restrict__ 
 struct SpmvTileRegs {
     int32_t r0, r1, b0, b1, ob0, ob1;
     bool has_blk, has_out;
-    double yz[3], yp[3], zo, po;
+    double yz[3], yp[3], zo, po, xo;
 };
 
 __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_info,
@@ -144,7 +148,8 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
                                       const int32_t *__restrict__ tcol,
                                       const double *__restrict__ tval,
                                       const double *__restrict__ z,
-                                      const double *__restrict__ p_old, SpmvTileRegs &T)
+                                      const double *__restrict__ p_old,
+                                      const double *__restrict__ x, SpmvTileRegs &T)
 {
     static_assert(kTileBlocks == kThreads, "one lane per padded block slot");
     // the block of this lane from its padded slot, loaded without waiting for
@@ -171,6 +176,7 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
     T.zo = z[o];
     T.po = p_old[o];
+    T.xo = x[o];
     const double z0 = z[j], z1 = z[j + 1], z2 = z[j + 2];
     const double q0 = p_old[j], q1 = p_old[j + 1], q2 = p_old[j + 2];
 #pragma unroll
@@ -186,6 +192,7 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
                                        const double *__restrict__ z,
                                        const double *__restrict__ p_old,
                                        double *__restrict__ p_new, double *__restrict__ q,
+                                       double *__restrict__ x, bool apply_x, double alpha_prev,
                                        double *ys)
 {
     const int a = static_cast<int>(threadIdx.x) % 3;
@@ -228,6 +235,7 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
         const double pi = T.zo + beta * T.po;
         p_new[o] = pi;
         q[o] = acc;
+        if (apply_x) x[o] = T.xo + alpha_prev * T.po;
         pq = pi * acc;
     }
     return pq;
@@ -244,7 +252,8 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
                 const double *__restrict__ p_old, double *__restrict__ p_new,
                 double *__restrict__ q, int32_t k, double tol_rel,
                 const double *__restrict__ pr_cur, int32_t np_cur,
-                double *__restrict__ rz_hist, double *__restrict__ pq_out, PcgStatus *status)
+                double *__restrict__ rz_hist, double *__restrict__ pq_out,
+                double *__restrict__ x, const double *__restrict__ alpha_hist, PcgStatus *status)
 {
     __shared__ double ys[3 * kTileBlocks];
     __shared__ double red[2][kThreads / kWave];
@@ -265,6 +274,10 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     const int32_t done_in = status->done;
     const double tol_in = status->tol;
     const double rz_prev = k > 0 ? rz_hist[(k + 1) & 1] : 1.0;
+    // x_k = x_(k-1) + alpha_(k-1) p_(k-1) is this launch's to apply when B(k-1)
+    // computed alpha (x_iter == k), whether or not the solve stops here
+    const bool apply_x = status->x_iter == k;
+    const double alpha_prev = k > 0 ? alpha_hist[(k + 1) & 1] : 0.0;
     // (unconditional loads from clamped, in-range indices: a predicated load
     // would be waited for where its branch joins)
     const int tid = static_cast<int>(threadIdx.x);
@@ -272,13 +285,24 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
     const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
-    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, tcol, tval, z, p_old, T);
+    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
     const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     double red_v[2];
     red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
     red_v[1] = (0.0 + (in_a ? rz_a : 0.0)) + (in_b ? rz_b : 0.0);
     block_allsum<2, kThreads>(red_v, red);
-    if (done_in) return;
+    // the pending x update of this workgroup's rows, for launches that stop
+    auto apply_x_only = [&]() {
+        for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
+            const int4 ti = tile_info[t];
+            const int64_t o = 3 * static_cast<int64_t>(ti.x) + threadIdx.x;
+            if (o < 3 * static_cast<int64_t>(ti.y)) x[o] = x[o] + alpha_prev * p_old[o];
+        }
+    };
+    if (done_in) {
+        if (apply_x) apply_x_only();
+        return;
+    }
     const double rr = red_v[0], rz = red_v[1];
     double tol;
     if (k == 0) {
@@ -297,6 +321,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
                 status->iters = k;
                 status->rnorm = sqrt(rr);
             }
+            if (apply_x) apply_x_only();
             return;
         }
     }
@@ -306,9 +331,9 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
         if (t != t0) {
             __syncthreads();  // the previous tile is summed before ys is rewritten
-            spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, T);
+            spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
         }
-        pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, ys);
+        pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, x, apply_x, alpha_prev, ys);
     }
     write_partials<1>(pq, pq_out, kSpmvMaxGrid, t0);  // slot t0: the sum of tiles t0, t0 + grid, ...
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -323,24 +348,21 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
 // loads (x, r, p, q, D^-1) are issued before the p.q partials are loaded and
 // reduced; D^-1 is stored column-major per entry ([c][3i+a]).
 __global__ void __launch_bounds__(kThreads)
-pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
-                  const double *__restrict__ dinv, double *__restrict__ x,
-                  double *__restrict__ r, double *__restrict__ z, int32_t n,
+pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
+                  double *__restrict__ r, double *__restrict__ z, int32_t n, int32_t k,
                   const double *__restrict__ pq_in, int32_t np_pq,
                   const double *__restrict__ rz_cur, double *__restrict__ pr_out,
-                  PcgStatus *status)
+                  double *__restrict__ alpha_out, PcgStatus *status)
 {
     __shared__ double red[1][kThreads / kWave];
     const int64_t dim = 3 * static_cast<int64_t>(n);
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    double xp[3], rp[3], pp[3], qp[3], di[9];
+    double rp[3], qp[3], di[9];
     auto load = [&](int32_t ii) {
         const int64_t o = 3 * static_cast<int64_t>(ii);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            xp[c] = x[o + c];
             rp[c] = r[o + c];
-            pp[c] = p[o + c];
             qp[c] = q[o + c];
         }
 #pragma unroll
@@ -363,6 +385,10 @@ pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
         return;
     }
     const double alpha = rz / red_v[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *alpha_out = alpha;
+        status->x_iter = k + 1;
+    }
     double v[2] = {0.0, 0.0};
     for (; i < n; i += gridDim.x * blockDim.x) {
         if (i != static_cast<int32_t>(blockIdx.x * blockDim.x + threadIdx.x)) load(i);
@@ -370,7 +396,6 @@ pcg_update_kernel(const double *__restrict__ p, const double *__restrict__ q,
         double rr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            x[o + c] = xp[c] + alpha * pp[c];
             rr[c] = rp[c] - alpha * qp[c];
             r[o + c] = rr[c];
         }
@@ -405,6 +430,20 @@ pcg_finalize_kernel(const double *__restrict__ pr_cur, int32_t np_cur, int32_t k
             status->done = 1;
         }
     }
+}
+
+// After the last launched iteration K: x_K = x_(K-1) + alpha_(K-1) p_(K-1)
+// when no A(K) ran to apply it (cap, or a stop found by the finalize test).
+__global__ void __launch_bounds__(kThreads)
+pcg_xfinish_kernel(double *__restrict__ x, const double *__restrict__ p_last,
+                   const double *__restrict__ alpha_last, int64_t m, int32_t k,
+                   const PcgStatus *status)
+{
+    if (status->x_iter != k) return;
+    const double alpha = *alpha_last;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < m;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        x[i] = x[i] + alpha * p_last[i];
 }
 
 // y = A x (FP64) and, if b != NULL, ||b - y||^2 / ||b||^2 partials.
@@ -569,12 +608,12 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
                 launch(pcg_spmv_kernel, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval,
                        g.n_tiles, g.z,
                        pold, pnew, g.q, k, o.tol, PR[k & 1], static_cast<int32_t>(ge),
-                       g.part + kPartOffRz, PQ, st);
+                       g.part + kPartOffRz, PQ, x_out, g.part + kPartOffAlpha, st);
             };
             auto update_args = [&](auto launch) {
-                launch(pcg_update_kernel, pnew, g.q, g.dinv, x_out, g.r, g.z, n, PQ,
+                launch(pcg_update_kernel, g.q, g.dinv, g.r, g.z, n, k, PQ,
                        static_cast<int32_t>(gt), g.part + kPartOffRz + (k & 1),
-                       PR[(k + 1) & 1], st);
+                       PR[(k + 1) & 1], g.part + kPartOffAlpha + (k & 1), st);
             };
             const dim3 blk(kThreads);
             if (ev) {
@@ -598,6 +637,11 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
                            static_cast<int32_t>(ge), k, o.max_iter, st);
     };
     GSLAM_TRY(run_chunked(g, o, enqueue_iters, k));
+    if (k > 0)
+        hipLaunchKernelGGL(pcg_xfinish_kernel, dim3(grid_for(3 * static_cast<int64_t>(n), 1)),
+                           dim3(kThreads), 0, g.stream, x_out, g.p[k & 1],
+                           g.part + kPartOffAlpha + ((k - 1) & 1), 3 * static_cast<int64_t>(n), k,
+                           st);
     GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
     GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
@@ -673,12 +717,12 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
                            g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.n_tiles, g.z, g.p[0],
                            g.p[1], g.q, 0,
                            1e-30, PR[0], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ,
-                           g.status);
+                           g.x, g.part + kPartOffAlpha, g.status);
     };
     auto update = [&]() {
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.p[1],
-                           g.q, g.dinv, g.x, g.r, g.z, n, PQ, static_cast<int32_t>(gt),
-                           g.part + kPartOffRz, PR[1], g.status);
+        hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.q,
+                           g.dinv, g.r, g.z, n, 0, PQ, static_cast<int32_t>(gt),
+                           g.part + kPartOffRz, PR[1], g.part + kPartOffAlpha, g.status);
     };
     auto run = [&](int rr) {
         for (int i = 0; i < rr; ++i) {

@@ -23,6 +23,7 @@ constexpr int kPartOffResid = 6 * kPartStride;  // residual / update norms (2 ar
 constexpr int kSpmvMaxGrid = 4096;
 constexpr int kPartOffPQT = 8 * kPartStride;    // p.q of the tiled SpMV
 constexpr int kPartOffRz = 8 * kPartStride + kSpmvMaxGrid;  // reduced r.z by parity (2)
+constexpr int kPartOffAlpha = kPartOffRz + 2;                // alpha by parity (2)
 constexpr int kPartTotal = kPartOffRz + 8;
 
 constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
