@@ -64,6 +64,41 @@ int DeviceGraph::csr_tmp_reserve()
     return GSLAM_OK;
 }
 
+int DeviceGraph::tval_reserve()
+{
+    if (tval) return GSLAM_OK;
+    return alloc(&tval, static_cast<int64_t>(9 * kTileBlocks) * n_tiles);
+}
+
+int DeviceGraph::mixed_factors()
+{
+    if (factors_valid) return GSLAM_OK;
+    if (!from_edges) {
+        set_error("mixed PCG needs the edge factors of gslam_assemble (system was set from CSR)");
+        return GSLAM_ERR_STATE;
+    }
+    if (!frec) {
+        const int64_t N = h.n, dim = 3 * N, E = h.n_e;
+        GSLAM_TRY(alloc(&dinv32, 9 * N));
+        GSLAM_TRY(alloc(&r32, dim));
+        GSLAM_TRY(alloc(&z32, dim));
+        GSLAM_TRY(alloc(&q32, dim));
+        GSLAM_TRY(alloc(&p32[0], dim));
+        GSLAM_TRY(alloc(&p32[1], dim));
+        GSLAM_TRY(alloc(&xoth, 2 * E));
+        GSLAM_TRY(alloc(&xpos, 2 * E));
+        GSLAM_TRY(upload(xoth, h.xoth.data(), 2 * E * 4));
+        GSLAM_TRY(upload(xpos, h.xpos.data(), 2 * E * 4));
+        float *f = nullptr;
+        GSLAM_TRY(alloc(&f, 16 * E));
+        frec = f;  // last: frec != nullptr means every mixed buffer exists
+    }
+    GSLAM_TRY(launch_assembly(*this));
+    GSLAM_HIP(hipStreamSynchronize(stream));
+    factors_valid = true;
+    return GSLAM_OK;
+}
+
 int DeviceGraph::upload(void *dst, const void *src, int64_t nbytes)
 {
     if (nbytes <= 0) return GSLAM_OK;
@@ -133,7 +168,6 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
     GSLAM_TRY(g.alloc(&g.tile_info, g.n_tiles));
     GSLAM_TRY(g.alloc(&g.tcol, static_cast<int64_t>(kTileBlocks) * g.n_tiles));
-    GSLAM_TRY(g.alloc(&g.tval, static_cast<int64_t>(9 * kTileBlocks) * g.n_tiles));
     GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
     GSLAM_TRY(g.alloc(&g.xi, dim));
@@ -144,15 +178,6 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.p[0], dim));
     GSLAM_TRY(g.alloc(&g.p[1], dim));
     GSLAM_TRY(g.alloc(&g.dinv, 9 * N));
-    GSLAM_TRY(g.alloc(&g.dinv32, 9 * N));
-    GSLAM_TRY(g.alloc(&g.r32, dim));
-    GSLAM_TRY(g.alloc(&g.z32, dim));
-    GSLAM_TRY(g.alloc(&g.q32, dim));
-    GSLAM_TRY(g.alloc(&g.p32[0], dim));
-    GSLAM_TRY(g.alloc(&g.p32[1], dim));
-    GSLAM_TRY(g.alloc(&g.xoth, 2 * E));
-    GSLAM_TRY(g.alloc(&g.xpos, 2 * E));
-    GSLAM_TRY(g.alloc(&g.frec, 16 * E));
     GSLAM_TRY(g.alloc(&g.part, kPartTotal));
     GSLAM_TRY(g.alloc(&g.status, 1));
     GSLAM_TRY(g.alloc(&g.scal, 16));
@@ -192,8 +217,6 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
         }
         GSLAM_TRY(g.upload(g.tcol, tc.data(), static_cast<int64_t>(tc.size()) * 4));
     }
-    GSLAM_TRY(g.upload(g.xoth, h.xoth.data(), 2 * E * 4));
-    GSLAM_TRY(g.upload(g.xpos, h.xpos.data(), 2 * E * 4));
     return GSLAM_OK;
 }
 
@@ -326,7 +349,8 @@ int gslam_assemble(gslam_graph *gh)
     GSLAM_TRY(gslam::launch_assembly(g));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
     g.assembled = true;
-    g.factors_valid = true;
+    g.from_edges = true;
+    g.factors_valid = g.frec != nullptr;
     g.solved = false;
     return GSLAM_OK;
 }
@@ -343,6 +367,7 @@ int gslam_set_system_csr(gslam_graph *gh, const double *values, const double *xi
     (void)hipStreamSynchronize(g.stream);
     if (rc != GSLAM_OK) return rc;
     g.assembled = true;
+    g.from_edges = false;
     g.factors_valid = false;
     g.solved = false;
     return GSLAM_OK;
@@ -434,10 +459,7 @@ static int solve_into(DeviceGraph &g, const gslam_solve_opts &o, const double *b
     if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED ||
         o.solver == GSLAM_SOLVER_GJ_BLOCKED64)
         return gslam::dense_solve(g, o, b, x, s);
-    if (o.solver == GSLAM_SOLVER_PCG_MIXED && !g.factors_valid) {
-        set_error("mixed PCG needs the edge factors of gslam_assemble (system was set from CSR)");
-        return GSLAM_ERR_STATE;
-    }
+    if (o.solver == GSLAM_SOLVER_PCG_MIXED) GSLAM_TRY(g.mixed_factors());
     GSLAM_TRY(gslam::pcg_solve(g, o, b, x, s));
     if (!s.converged && !(s.flags & 1) && o.max_iter > 0 && s.iters >= o.max_iter)
         return GSLAM_ERR_NOT_CONVERGED;

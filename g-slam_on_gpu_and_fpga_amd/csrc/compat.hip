@@ -121,6 +121,8 @@ void device_pcg_solve(double *x_h, double *d_csr_vals, double *d_xi, int dim, in
                              g.stream), "copy result to host");
     hip_check(hipStreamSynchronize(g.stream), "copy result to host");
     g.assembled = true;
+    g.from_edges = false;
+    g.factors_valid = false;
     g.solved = true;
 }
 

@@ -83,6 +83,7 @@ struct DeviceGraph {
     int64_t bytes = 0;
     bool assembled = false, solved = false;
     bool factors_valid = false;  // frec matches the assembled system
+    bool from_edges = false;     // the assembled system came from gslam_assemble
 
     // inputs / poses
     double *poses = nullptr;     // 3N
@@ -101,7 +102,8 @@ struct DeviceGraph {
     // (column index; a padding slot repeats the tile's first column), so an
     // SpMV lane finds its block without the tile record: tcol fixed, tval
     // packed from bval at the start of each FP64 solve (zero in padding),
-    // entry k of slot l of tile t at (9 t + k) kTileBlocks + l.
+    // entry k of slot l of tile t at (9 t + k) kTileBlocks + l.  tval is
+    // allocated by the first FP64 PCG solve (tval_reserve).
     int32_t *tcol = nullptr;
     double *tval = nullptr;
 
@@ -127,6 +129,8 @@ struct DeviceGraph {
     // factorized FP32 operator of the mixed solver: per incidence entry k
     // (pose i, incident edge e, side a; layout of xlist) the other pose
     // xoth[k] and 8 FP32 edge factors frec[8k..8k+7]; xpos[2e+a] = k.
+    // These and the FP32 vectors above are allocated by the first mixed
+    // solve (mixed_factors); until then the assembly kernels skip frec.
     int32_t *xoth = nullptr, *xpos = nullptr;
     float *frec = nullptr;
 
@@ -138,6 +142,11 @@ struct DeviceGraph {
     double *csr_tmp = nullptr;
     int vecs_reserve();
     int csr_tmp_reserve();
+    int tval_reserve();
+    // Makes frec valid for the assembled system: allocates the mixed solver's
+    // buffers on first use and, if the last assembly ran without them,
+    // re-runs it (the assembly is deterministic: bval and xi keep their bits).
+    int mixed_factors();
     int64_t dim() const { return 3 * static_cast<int64_t>(h.n); }
 
     DenseWork dense;             // GJ_MIXED work space (allocated on first use)

@@ -522,12 +522,14 @@ diff_kernel(const double *__restrict__ part, int32_t np, int32_t n, double *__re
 }
 
 // The padded tile copy of the current block values (before each FP64 solve).
-static void launch_tile_pack(DeviceGraph &g)
+static int launch_tile_pack(DeviceGraph &g)
 {
+    GSLAM_TRY(g.tval_reserve());
     const int64_t n = static_cast<int64_t>(g.n_tiles) * kTileBlocks * 9;
     const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(4096, (n + kThreads - 1) / kThreads));
     hipLaunchKernelGGL(tile_pack_kernel, dim3(grid), dim3(kThreads), 0, g.stream, g.tile_info, g.bval,
                        g.n_tiles, g.tval);
+    return GSLAM_OK;
 }
 
 int launch_spmv64(DeviceGraph &g, const double *x, double *y)
@@ -585,7 +587,7 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
     double *PQ = g.part + kPartOffPQT;
     PcgStatus *st = g.status;
     GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
-    launch_tile_pack(g);
+    GSLAM_TRY(launch_tile_pack(g));
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.bdiag, g.bval, b,
                        n, o.max_iter, g.dinv, static_cast<float *>(nullptr), x_out, g.r, g.z,
                        g.p[0], g.p[1], PR[0], st);
@@ -732,7 +734,7 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
         }
     };
     // a consistent solver state: setup, one SpMV (valid p.q partials)
-    launch_tile_pack(g);
+    GSLAM_TRY(launch_tile_pack(g));
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        g.xi, n, 1, g.dinv, static_cast<float *>(nullptr), g.x, g.r, g.z, g.p[0],
                        g.p[1], PR[0], g.status);

@@ -602,10 +602,7 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
 
 int time_mixed_spmv(DeviceGraph &g, int reps, double *us)
 {
-    if (!g.factors_valid) {
-        set_error("mixed operator timing needs gslam_assemble");
-        return GSLAM_ERR_STATE;
-    }
+    GSLAM_TRY(g.mixed_factors());
     const int32_t n = g.h.n;
     const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};

@@ -281,6 +281,13 @@ class Graph:
         _check(L.gslam_get_info(self.h, C.byref(info)), "gslam_get_info")
         self.info = {k: getattr(info, k) for k, _ in _Info._fields_}
 
+    def device_bytes(self) -> int:
+        """Device memory the handle holds now: `info["device_bytes"]` is the
+        figure at creation; solver work space is added on its first use."""
+        info = _Info()
+        _check(lib().gslam_get_info(self.h, C.byref(info)), "gslam_get_info")
+        return int(info.device_bytes)
+
     @classmethod
     def from_log(cls, path: str | None = None, lam: float = 0.1, pair_order: int = 0,
                  wrap: int = 0, data: dict | None = None, anchor_prior: int = 0):
