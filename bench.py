@@ -67,16 +67,16 @@ def log(msg):
 def spmv_bytes(n_poses, n_blocks):
     """Algorithmic bytes of one FP64 PCG SpMV launch: the BSR values and
     column indices (9*8 + 4 B per block), block-row pointers (4 B per row),
-    z and p_old read once (2 x 24 B per row) and p_new, q written once
-    (2 x 24 B per row)."""
-    return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 4 * 24 * n_poses
+    z, p_old and x read once (3 x 24 B per row) and p_new, q and x (the
+    previous iteration's x += alpha p_old) written once (3 x 24 B per row)."""
+    return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 6 * 24 * n_poses
 
 
 def update_bytes(n_poses):
-    """Algorithmic bytes of one FP64 PCG update launch: x, r, p, q and the
-    3x3 block of D^-1 read (4 x 24 + 72 B per pose), x, r and z written (3 x 24
-    B per pose)."""
-    return n_poses * (4 * 24 + 72 + 3 * 24)
+    """Algorithmic bytes of one FP64 PCG update launch: r, q and the 3x3 block
+    of D^-1 read (2 x 24 + 72 B per pose), r and z written (2 x 24 B per
+    pose)."""
+    return n_poses * (2 * 24 + 72 + 2 * 24)
 
 
 def mixed_spmv_bytes(n_poses, n_inc):
@@ -134,7 +134,7 @@ def pcg_section(args, gslam, G, np):
     log(f"graph N={args.n}: {len(gc['i'])} edges ({time.time() - t0:.1f}s)")
     init = gc["poses"].copy()
     g = gslam.Graph(init, **G.graph_kwargs(gc, 0.1, wrap=1))
-    info = g.info
+    info = dict(g.info)
     log(f"handle: {info}")
     dx_ref = dx_pcg = None
     if not args.no_accuracy:
@@ -183,7 +183,8 @@ def pcg_section(args, gslam, G, np):
         hip_sync()
         ms = 1000.0 * (time.perf_counter() - t_start) / args.steps
         results[solver] = dict(ms_per_step=ms, tol=tol, pcg_iters=int(np.mean(iters)),
-                               us_per_iter=1000.0 * ms / max(np.mean(iters), 1), **acc)
+                               us_per_iter=1000.0 * ms / max(np.mean(iters), 1),
+                               handle_device_mb=g.device_bytes() / 1e6, **acc)
         log(f"{solver}: {ms:.1f} ms per GN step, {np.mean(iters):.0f} PCG iterations")
 
     # dominant kernel: FP64 SpMV, mean duration inside the timed solves
