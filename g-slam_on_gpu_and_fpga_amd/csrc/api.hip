@@ -79,18 +79,37 @@ int DeviceGraph::mixed_factors()
     }
     if (!frec) {
         const int64_t N = h.n, dim = 3 * N, E = h.n_e;
+        MixedTiles m;
+        std::string err;
+        const int rc = build_mixed_tiles(h, m, err);
+        if (rc != GSLAM_OK) {
+            set_error(err);
+            return rc;
+        }
+        const int64_t slots = m.n_chunks * kTileBlocks;
         GSLAM_TRY(alloc(&dinv32, 9 * N));
         GSLAM_TRY(alloc(&r32, dim));
         GSLAM_TRY(alloc(&z32, dim));
         GSLAM_TRY(alloc(&q32, dim));
         GSLAM_TRY(alloc(&p32[0], dim));
         GSLAM_TRY(alloc(&p32[1], dim));
-        GSLAM_TRY(alloc(&xoth, 2 * E));
-        GSLAM_TRY(alloc(&xpos, 2 * E));
-        GSLAM_TRY(upload(xoth, h.xoth.data(), 2 * E * 4));
-        GSLAM_TRY(upload(xpos, h.xpos.data(), 2 * E * 4));
+        GSLAM_TRY(alloc(&mt_info, m.n_tiles));
+        GSLAM_TRY(alloc(&mt_ext, m.n_tiles));
+        GSLAM_TRY(alloc(&mt_ent, slots));
+        GSLAM_TRY(alloc(&mt_own, slots));
+        GSLAM_TRY(alloc(&mt_oth, slots));
+        GSLAM_TRY(alloc(&mt_pos, 2 * E));
+        GSLAM_TRY(upload(mt_info, m.info.data(), static_cast<int64_t>(m.info.size()) * 4));
+        GSLAM_TRY(upload(mt_ext, m.ext.data(), static_cast<int64_t>(m.n_tiles) * 4));
+        GSLAM_TRY(upload(mt_ent, m.ent.data(), slots * 4));
+        GSLAM_TRY(upload(mt_own, m.own.data(), slots * 4));
+        GSLAM_TRY(upload(mt_oth, m.oth.data(), slots * 4));
+        GSLAM_TRY(upload(mt_pos, m.pos.data(), 2 * E * 4));
         float *f = nullptr;
-        GSLAM_TRY(alloc(&f, 16 * E));
+        GSLAM_TRY(alloc(&f, 8 * slots));
+        // padding slots keep zero factors
+        GSLAM_HIP(hipMemsetAsync(f, 0, static_cast<size_t>(8 * slots) * 4, stream));
+        mt_tiles = m.n_tiles;
         frec = f;  // last: frec != nullptr means every mixed buffer exists
     }
     GSLAM_TRY(launch_assembly(*this));

@@ -16,18 +16,21 @@
 
 namespace gslam {
 
-// 8 FP32 factors of one edge, stored at both of its incidence entries (the
-// mixed solver's factorized operator, pcg_mixed.hip).
-__device__ inline void put_factors(float *__restrict__ frec, const int32_t *__restrict__ xpos,
+// 8 FP32 factors of one edge, stored at the slots of both of its incidence
+// entries (the mixed solver's factorized operator, pcg_mixed.hip): factor c
+// of slot 256 h + l at frec[(8 h + c) 256 + l].
+__device__ inline void put_factors(float *__restrict__ frec, const int32_t *__restrict__ mt_pos,
                                    int64_t e, float4 a, float4 b)
 {
     if (!frec) return;
-    float4 *f = reinterpret_cast<float4 *>(frec);
-    const int64_t k0 = xpos[2 * e], k1 = xpos[2 * e + 1];
-    f[2 * k0] = a;
-    f[2 * k0 + 1] = b;
-    f[2 * k1] = a;
-    f[2 * k1 + 1] = b;
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int64_t s = mt_pos[2 * e + side];
+        float *f = frec + (s / kTileBlocks) * 8 * kTileBlocks + s % kTileBlocks;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) f[c * kTileBlocks] = v[c];
+    }
 }
 
 // Velocity-model motion edge (i, i+1).  Model: src/cpu_pcg_fp64/MotionEdge.c
@@ -40,7 +43,7 @@ motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__
                     int64_t n_mot, int64_t e_off, double delta, double delta_t,
                     double lambda, double m0, double m1, double m2, double m3,
                     int wrap, double *__restrict__ stage, float *__restrict__ frec,
-                    const int32_t *__restrict__ xpos)
+                    const int32_t *__restrict__ mt_pos)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n_mot) return;
@@ -112,7 +115,7 @@ motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__
         s[39 + r] = -lambda * (Om[3 * r] * e0 + Om[3 * r + 1] * e1 + Om[3 * r + 2] * e2);
     }
     // factors: J1 = -F, J2 = I, W = lambda Om
-    put_factors(frec, xpos, e_off + i,
+    put_factors(frec, mt_pos, e_off + i,
                 make_float4(static_cast<float>(F02), static_cast<float>(F12),
                             static_cast<float>(lambda * Om[0]), static_cast<float>(lambda * Om[1])),
                 make_float4(static_cast<float>(lambda * Om[2]), static_cast<float>(lambda * Om[4]),
@@ -127,7 +130,7 @@ __global__ void __launch_bounds__(kThreads)
 obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
                  const int32_t *__restrict__ t2, const double *__restrict__ op,
                  int64_t n_obs_e, double snr0, double snr1, double *__restrict__ stage,
-                 float *__restrict__ frec, const int32_t *__restrict__ xpos)
+                 float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
 {
     const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (e >= n_obs_e) return;
@@ -177,7 +180,7 @@ obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t
     s[40] = -Oey;
     s[41] = -(b2x * Oex + b2y * Oey);
     // factors: B1 = [-I | b1], B2 = [I | b2], W = Om
-    put_factors(frec, xpos, e,
+    put_factors(frec, mt_pos, e,
                 make_float4(static_cast<float>(b1x), static_cast<float>(b1y),
                             static_cast<float>(b2x), static_cast<float>(b2y)),
                 make_float4(static_cast<float>(o00), static_cast<float>(o01),
@@ -192,7 +195,7 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
                      const int32_t *__restrict__ t2, const double *__restrict__ bp,
                      const double *__restrict__ bw, int64_t n_bet, int64_t e_off,
                      double lambda, int wrap, int exact, double *__restrict__ stage,
-                     float *__restrict__ frec, const int32_t *__restrict__ xpos)
+                     float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
 {
     const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k >= n_bet) return;
@@ -249,7 +252,7 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
         st[18 + 5] -= cy;
     }
     // factors: rotation (c, s), theta column (ga, gb), W = diag(L)
-    put_factors(frec, xpos, e,
+    put_factors(frec, mt_pos, e,
                 make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(ga),
                             static_cast<float>(gb)),
                 make_float4(static_cast<float>(L0), static_cast<float>(L1), static_cast<float>(L2), 0.f));
@@ -325,19 +328,19 @@ int launch_assembly(DeviceGraph &g, bool exact_hessian)
     if (h.n_obs_e > 0)
         hipLaunchKernelGGL(obs_edges_kernel, dim3(nblk(h.n_obs_e)), dim3(kThreads), 0, g.stream,
                            g.poses, g.e_t1, g.e_t2, g.obs_param, h.n_obs_e, g.snr[0], g.snr[1],
-                           g.stage, g.frec, g.xpos);
+                           g.stage, g.frec, g.mt_pos);
     if (h.n_mot_e > 0) {
         const double delta_t = static_cast<double>(static_cast<unsigned int>(g.delta));
         hipLaunchKernelGGL(motion_edges_kernel, dim3(nblk(h.n_mot_e)), dim3(kThreads), 0,
                            g.stream, g.poses, g.mot_param, h.n_mot_e, h.n_obs_e, g.delta,
                            delta_t, g.lambda, g.mns[0], g.mns[1], g.mns[2], g.mns[3], g.wrap,
-                           g.stage, g.frec, g.xpos);
+                           g.stage, g.frec, g.mt_pos);
     }
     if (h.n_bet_e > 0)
         hipLaunchKernelGGL(between_edges_kernel, dim3(nblk(h.n_bet_e)), dim3(kThreads), 0,
                            g.stream, g.poses, g.e_t1, g.e_t2, g.bet_param, g.bet_w, h.n_bet_e,
                            h.n_obs_e + h.n_mot_e, g.lambda, g.wrap, exact_hessian ? 1 : 0, g.stage,
-                           g.frec, g.xpos);
+                           g.frec, g.mt_pos);
     const int64_t nv = 9 * h.n_blocks;
     hipLaunchKernelGGL(gather_blocks_kernel, dim3(nblk(nv)), dim3(kThreads), 0, g.stream,
                        g.cptr, g.clist, g.stage, nv, static_cast<int64_t>(h.bdiag[0]), g.anchor,

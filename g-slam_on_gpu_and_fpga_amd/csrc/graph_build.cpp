@@ -210,17 +210,9 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
     g.xlist.resize(static_cast<size_t>(2 * g.n_e));
     {
         std::vector<int32_t> f(g.xptr.begin(), g.xptr.end() - 1);
-        g.xoth.resize(static_cast<size_t>(2 * g.n_e));
-        g.xpos.resize(static_cast<size_t>(2 * g.n_e));
         for (int64_t e = 0; e < g.n_e; ++e) {
-            const int32_t k0 = f[g.t1[e]]++;
-            g.xlist[k0] = static_cast<int32_t>(2 * e);
-            g.xoth[k0] = g.t2[e];
-            g.xpos[2 * e] = k0;
-            const int32_t k1 = f[g.t2[e]]++;
-            g.xlist[k1] = static_cast<int32_t>(2 * e + 1);
-            g.xoth[k1] = g.t1[e];
-            g.xpos[2 * e + 1] = k1;
+            g.xlist[f[g.t1[e]]++] = static_cast<int32_t>(2 * e);
+            g.xlist[f[g.t2[e]]++] = static_cast<int32_t>(2 * e + 1);
         }
     }
 
@@ -259,6 +251,67 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
         (void)cnt;
     }
     g.nnz = pos;
+    return GSLAM_OK;
+}
+
+int build_mixed_tiles(const HostGraph &g, MixedTiles &m, std::string &err)
+{
+    const int32_t N = g.n;
+    std::vector<int32_t> rows{0};
+    for (int32_t r0 = 0; r0 < N;) {
+        int32_t r1 = r0 + 1;
+        while (r1 < N && r1 - r0 < kTileRows && g.xptr[r1 + 1] - g.xptr[r0] <= kTileBlocks) ++r1;
+        rows.push_back(r1);
+        r0 = r1;
+    }
+    const int64_t T = static_cast<int64_t>(rows.size()) - 1;
+    m.ext.assign(static_cast<size_t>(T), -1);
+    int64_t chunks = T;
+    for (int64_t t = 0; t < T; ++t) {
+        const int64_t len = g.xptr[rows[t + 1]] - g.xptr[rows[t]];
+        if (len > kTileBlocks) {
+            m.ext[t] = static_cast<int32_t>(chunks);
+            chunks += (len - 1) / kTileBlocks;
+        }
+    }
+    if (chunks * kTileBlocks > std::numeric_limits<int32_t>::max())
+        return fail(err, GSLAM_ERR_TOO_LARGE, "mixed operator: more than 2^31 incidence slots");
+    m.n_tiles = static_cast<int32_t>(T);
+    m.n_chunks = chunks;
+    const size_t slots = static_cast<size_t>(chunks) * kTileBlocks;
+    m.info.resize(static_cast<size_t>(4 * T));
+    m.ent.assign(slots, -1);
+    m.own.assign(slots, 0);
+    m.oth.assign(slots, 0);
+    m.pos.assign(static_cast<size_t>(2 * g.n_e), -1);
+    for (int64_t t = 0; t < T; ++t) {
+        const int32_t r0 = rows[t], r1 = rows[t + 1], x0 = g.xptr[r0], x1 = g.xptr[r1];
+        m.info[4 * t] = r0;
+        m.info[4 * t + 1] = r1;
+        m.info[4 * t + 2] = x0;
+        m.info[4 * t + 3] = x1;
+        const int64_t n_ch = x1 - x0 > kTileBlocks ? 1 + (x1 - x0 - 1) / kTileBlocks : 1;
+        for (int64_t c = 0; c < n_ch; ++c) {
+            const int64_t chunk = c == 0 ? t : m.ext[t] + c - 1;
+            for (int l = 0; l < kTileBlocks; ++l) {
+                const size_t slot = static_cast<size_t>(chunk) * kTileBlocks + l;
+                m.own[slot] = r0;
+                m.oth[slot] = r0;
+            }
+        }
+        for (int32_t k = x0; k < x1; ++k) {
+            const int64_t j = k - x0;
+            const int64_t chunk = j < kTileBlocks ? t : m.ext[t] + j / kTileBlocks - 1;
+            const size_t slot = static_cast<size_t>(chunk) * kTileBlocks + j % kTileBlocks;
+            const int32_t ent = g.xlist[k];
+            const int64_t e = ent >> 1;
+            const bool from = (ent & 1) == 0;  // this pose is the edge's t1
+            m.ent[slot] = ent;
+            m.own[slot] = from ? g.t1[e] : g.t2[e];
+            m.oth[slot] = from ? g.t2[e] : g.t1[e];
+            m.pos[ent] = static_cast<int32_t>(slot);
+        }
+    }
     return GSLAM_OK;
 }
 

@@ -41,9 +41,6 @@ struct HostGraph {
     // (entries e*4+q in edge order); pose i receives xi halves
     // xlist[xptr[i]..xptr[i+1]) (entries e*2+side in edge order).
     std::vector<int32_t> cptr, clist, xptr, xlist;
-    // Per incidence entry k of xlist: the other pose of the edge; and the
-    // inverse map xpos[2e+side] = k.
-    std::vector<int32_t> xoth, xpos;
 
     // SpMV tiles: tile t covers block rows [tile_row[t], tile_row[t+1]),
     // whole rows, at most kTileBlocks blocks and kTileRows rows unless a
@@ -55,6 +52,24 @@ struct HostGraph {
     std::vector<int32_t> csr_row_ptr, csr_col_idx, csr_pos;
     int64_t nnz = 0;
 };
+
+// Tiles of the mixed solver's factorized FP32 operator (pcg_mixed.hip): the
+// incidence lists xlist cut into tiles of whole poses, greedy, at most
+// kTileBlocks entries and kTileRows poses unless one pose has more entries
+// (then it is a tile of its own).  Entry j of tile t sits in slot
+// 256 t + j (j < 256, "chunk" t); a longer pose continues in extra chunks
+// numbered from n_tiles on (ext[t] is its first, -1 for a one-chunk tile).
+// Per slot: the entry e*2+side (-1 in padding), its own pose and the other
+// pose of the edge (padding: the tile's first pose); pos[2e+side] = slot.
+struct MixedTiles {
+    int32_t n_tiles = 0;
+    int64_t n_chunks = 0;
+    std::vector<int32_t> info;  // 4 per tile: first pose, end pose, xptr of both
+    std::vector<int32_t> ext;   // per tile
+    std::vector<int32_t> ent, own, oth;  // per slot (256 per chunk)
+    std::vector<int32_t> pos;   // 2 per edge
+};
+int build_mixed_tiles(const HostGraph &g, MixedTiles &m, std::string &err);
 
 // Validates the description and builds everything above.  Returns GSLAM_OK
 // or a negative status with a message in *err.

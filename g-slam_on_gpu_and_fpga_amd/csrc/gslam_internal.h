@@ -126,12 +126,19 @@ struct DeviceGraph {
     double *scal_host = nullptr; // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_poll[2] = {nullptr, nullptr};
 
-    // factorized FP32 operator of the mixed solver: per incidence entry k
-    // (pose i, incident edge e, side a; layout of xlist) the other pose
-    // xoth[k] and 8 FP32 edge factors frec[8k..8k+7]; xpos[2e+a] = k.
-    // These and the FP32 vectors above are allocated by the first mixed
-    // solve (mixed_factors); until then the assembly kernels skip frec.
-    int32_t *xoth = nullptr, *xpos = nullptr;
+    // factorized FP32 operator of the mixed solver, in the tiles of
+    // build_mixed_tiles (graph_build.h): per tile {first pose, end pose,
+    // xptr of both} (mt_info) and its first extra chunk (mt_ext); per slot
+    // the incidence entry e*2+side (-1 in padding), its own and other pose;
+    // mt_pos[2e+side] = slot; the 8 FP32 factors of the slot's edge, stored
+    // plane-major per 256-slot chunk: factor c of slot 256 h + l at
+    // frec[(8 h + c) 256 + l].  These and the FP32 vectors above are
+    // allocated by the first mixed solve (mixed_factors); until then the
+    // assembly kernels skip frec.
+    int4 *mt_info = nullptr;
+    int32_t *mt_ext = nullptr, *mt_ent = nullptr, *mt_own = nullptr, *mt_oth = nullptr;
+    int32_t *mt_pos = nullptr;
+    int32_t mt_tiles = 0;
     float *frec = nullptr;
 
     // Vector work space, allocated on first use and kept: 4 x 3N + n_between

@@ -30,14 +30,14 @@
 
 namespace gslam {
 
+// Lanes per pose row of the FP64 residual kernels below.
 constexpr int kLanesMixed = 4;
 
-// One group of kLanesMixed lanes per pose row, one pass (up to kSpmvMaxGrid
-// workgroups, grid-stride beyond); p.q partials in the kSpmvMaxGrid array.
-static inline unsigned int mixed_spmv_grid(int32_t n)
+// One workgroup per operator tile, one pass (up to kSpmvMaxGrid workgroups,
+// tile-stride beyond); p.q partials in the kSpmvMaxGrid array.
+static inline unsigned int mixed_spmv_grid(const DeviceGraph &g)
 {
-    int64_t g = (static_cast<int64_t>(n) * kLanesMixed + kThreads - 1) / kThreads;
-    return static_cast<unsigned int>(std::max<int64_t>(1, std::min<int64_t>(g, kSpmvMaxGrid)));
+    return static_cast<unsigned int>(std::max<int32_t>(1, std::min<int32_t>(g.mt_tiles, kSpmvMaxGrid)));
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -162,74 +162,120 @@ __device__ inline void edge_apply(int type, int side, const float4 f0, const flo
 }
 
 // A(k): stopping test (true residual slots only), beta, q = Omega_32 p with
-// p = z + beta p_old on the fly, p.q partials.
+// p = z + beta p_old formed on the fly, p.q partials.
 //
-// Latency-bound at the sizes of interest: the row's own z / p_old and, per
-// lane, its first incidence entry (index, other pose, the two factor
-// records, the other pose's z / p_old) are loaded before the r.r / r.z
-// partials are reduced (one barrier); further entries of long rows are read
-// after.  The arithmetic is the same as forming each p on the fly.
-template <int G>
+// One workgroup per tile of whole poses (<= 256 incidence entries, <= 85
+// poses; the tiles of build_mixed_tiles): one lane per incidence entry (its
+// entry, own and other pose from slot 256 t + lane -- no wait for the tile
+// record -- and its edge's 8 factors, plane-major, one 1 KB run per factor
+// and workgroup), the entry's 3 contributions to its own pose staged in LDS,
+// then one thread per output row (3 per pose) sums its pose's entries in
+// entry order.  Issue order, as in the FP64 SpMV: the r.r / r.z partials
+// that give beta and the stop flag, the slot, the tile record and the output
+// row's range and z / p_old, then the gathered z / p_old of both poses -- two
+// dependent loads deep; the partials are reduced (one barrier) while the
+// gathers are in flight.  A pose with more than 256 entries is a tile of its
+// own and continues in extra chunks, summed one chunk at a time.
+struct MixedSlotRegs {
+    int32_t ent;
+    float4 f0, f1;
+    float zi[3], qi[3], zo[3], qo[3];
+};
+
+__device__ inline void mixed_slot_load(int64_t slot, const int32_t *__restrict__ mt_ent,
+                                       const int32_t *__restrict__ mt_own,
+                                       const int32_t *__restrict__ mt_oth,
+                                       const float *__restrict__ frec, const float *__restrict__ z,
+                                       const float *__restrict__ p_old, MixedSlotRegs &S)
+{
+    const int64_t oi = 3 * static_cast<int64_t>(mt_own[slot]);
+    const int64_t oo = 3 * static_cast<int64_t>(mt_oth[slot]);
+    S.ent = mt_ent[slot];
+    const float *f = frec + (slot / kTileBlocks) * 8 * kTileBlocks + slot % kTileBlocks;
+    S.f0 = make_float4(f[0], f[kTileBlocks], f[2 * kTileBlocks], f[3 * kTileBlocks]);
+    S.f1 = make_float4(f[4 * kTileBlocks], f[5 * kTileBlocks], f[6 * kTileBlocks],
+                       f[7 * kTileBlocks]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        S.zi[c] = z[oi + c];
+        S.qi[c] = p_old[oi + c];
+        S.zo[c] = z[oo + c];
+        S.qo[c] = p_old[oo + c];
+    }
+}
+
+// The entry's contributions to its own pose, into LDS column `col` (zeros
+// for a padding slot).
+__device__ inline void mixed_slot_apply(const MixedSlotRegs &S, float beta, int64_t n_obs_e,
+                                        int64_t n_mot_e, float *ys, int col)
+{
+    float y[3] = {0.f, 0.f, 0.f};
+    if (S.ent >= 0) {
+        const int64_t e = S.ent >> 1;
+        const int side = S.ent & 1;
+        const int type = e < n_obs_e ? 0 : (e < n_obs_e + n_mot_e ? 1 : 2);
+        float pi[3], po[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pi[c] = S.zi[c] + beta * S.qi[c];
+            po[c] = S.zo[c] + beta * S.qo[c];
+        }
+        edge_apply(type, side, S.f0, S.f1, pi, po, y);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ys[a * kTileBlocks + col] = y[a];
+}
+
 __global__ void __launch_bounds__(kThreads)
-mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ xlist,
-                  const int32_t *__restrict__ xoth, const float4 *__restrict__ frec,
-                  int64_t n_obs_e, int64_t n_mot_e, float anchor, const float *__restrict__ z,
+mixed_spmv_kernel(const int4 *__restrict__ mt_info, const int32_t *__restrict__ mt_ext,
+                  const int32_t *__restrict__ mt_ent, const int32_t *__restrict__ mt_own,
+                  const int32_t *__restrict__ mt_oth, const float *__restrict__ frec,
+                  int32_t n_tiles, const int32_t *__restrict__ xptr, int64_t n_obs_e,
+                  int64_t n_mot_e, float anchor, const float *__restrict__ z,
                   const float *__restrict__ p_old, float *__restrict__ p_new,
-                  float *__restrict__ q, int32_t n, int32_t k, int32_t is_true, double tol_rel,
+                  float *__restrict__ q, int32_t k, int32_t is_true, double tol_rel,
                   const double *__restrict__ pr_cur, int32_t np_cur,
                   const double *__restrict__ pr_prev, int32_t np_prev,
                   double *__restrict__ pq_out, PcgStatus *status)
 {
+    static_assert(kTileBlocks == kThreads, "one lane per slot of a 256-slot chunk");
+    __shared__ float ys[3 * kTileBlocks];
     __shared__ double red[3][kThreads / kWave];
     static_assert(kMaxGrid <= 2 * kThreads, "two partials per thread and array");
-    constexpr int kRowsPerTile = kThreads / G;
-    const int lane = threadIdx.x % G;
-    // row group of this workgroup: in contiguous runs per XCD when one pass
-    // covers all rows (neighbouring rows share an L2), else the round robin
+    const int tid = static_cast<int>(threadIdx.x);
+    const int a = tid % 3;
+    // tiles in contiguous runs per XCD when every tile has a workgroup
+    // (neighbouring poses share an L2), else the round robin
     const int32_t g = static_cast<int32_t>(gridDim.x), b = static_cast<int32_t>(blockIdx.x);
-    const int32_t bx = (n + kRowsPerTile - 1) / kRowsPerTile <= g ? xcd_contiguous(b, g) : b;
-    const int64_t row_first = static_cast<int64_t>(bx) * kRowsPerTile + threadIdx.x / G;
+    const int32_t t0 = n_tiles <= g ? xcd_contiguous(b, g) : b;
     // The r.r / r.z partials of this and the previous parity first
     // (np_cur, np_prev <= 2 x kThreads: two per thread and array, summed below
     // in strided_sum's order), as unconditional loads from clamped indices and
-    // pinned here, so that their round trips overlap the row's loads.
-    const int tid = static_cast<int>(threadIdx.x);
+    // pinned here, so that their round trips overlap the tile's loads.
     const int ia = min(tid, np_cur - 1), ib = min(tid + kThreads, np_cur - 1);
     const int ja = min(tid, np_prev - 1), jb = min(tid + kThreads, np_prev - 1);
     const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
     const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
     const double zp_a = pr_prev[kPartStride + ja], zp_b = pr_prev[kPartStride + jb];
-    asm volatile("" ::: "memory");
-    // loads of the first row handled by this thread (independent of beta)
-    float zi[3] = {0.f, 0.f, 0.f}, qi[3] = {0.f, 0.f, 0.f};
-    float zo[3] = {0.f, 0.f, 0.f}, qo[3] = {0.f, 0.f, 0.f};
-    float4 f0 = make_float4(0.f, 0.f, 0.f, 0.f), f1 = f0;
-    int32_t e1 = 0, kk0 = 0, ent0 = 0;
-    if (row_first < n) {
-        const int64_t o = 3 * row_first;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            zi[c] = z[o + c];
-            qi[c] = p_old[o + c];
-        }
-        kk0 = xptr[row_first] + lane;
-        e1 = xptr[row_first + 1];
-        if (kk0 < e1) {
-            ent0 = xlist[kk0];
-            const int64_t oo = 3 * static_cast<int64_t>(xoth[kk0]);
-            f0 = frec[2 * kk0];
-            f1 = frec[2 * kk0 + 1];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                zo[c] = z[oo + c];
-                qo[c] = p_old[oo + c];
-            }
-        }
-    }
-    // the stop flag and tolerance as earlier launches left them, loaded ahead
-    // of the reduction's barrier (a load after it would add a round trip)
     const int32_t done_in = status->done;
     const double tol_in = status->tol;
+    asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
+    MixedSlotRegs S;
+    int4 ti = make_int4(0, 0, 0, 0);
+    int32_t ob0 = 0, ob1 = 0;
+    float zr = 0.f, pr = 0.f;
+    auto tile_load = [&](int32_t t) {
+        mixed_slot_load(static_cast<int64_t>(t) * kTileBlocks + tid, mt_ent, mt_own, mt_oth, frec,
+                        z, p_old, S);
+        ti = mt_info[t];
+        const int32_t orow = min(ti.x + tid / 3, ti.y - 1);
+        ob0 = xptr[orow];
+        ob1 = xptr[orow + 1];
+        const int64_t o = 3 * static_cast<int64_t>(orow) + a;
+        zr = z[o];
+        pr = p_old[o];
+    };
+    if (t0 < n_tiles) tile_load(t0);
     const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     const bool jn_a = tid < np_prev, jn_b = tid + kThreads < np_prev;
     double red_v[3];
@@ -260,62 +306,47 @@ mixed_spmv_kernel(const int32_t *__restrict__ xptr, const int32_t *__restrict__ 
     }
     const float beta = k > 0 ? static_cast<float>(red_v[1] / red_v[2]) : 0.f;
     double pq[1] = {0.0};
-    for (int64_t row0 = static_cast<int64_t>(bx) * kRowsPerTile; row0 < n;
-         row0 += static_cast<int64_t>(gridDim.x) * kRowsPerTile) {
-        const int64_t row = row0 + threadIdx.x / G;
-        const bool first = row == row_first;
-        float y[3] = {0.f, 0.f, 0.f};
-        float pi[3] = {0.f, 0.f, 0.f};
-        if (row < n) {
-            const int64_t o = 3 * row;
-            if (first) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) pi[c] = zi[c] + beta * qi[c];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) pi[c] = z[o + c] + beta * p_old[o + c];
-            }
-            const int32_t kk_start = first ? kk0 : xptr[row] + lane;
-            const int32_t kk_end = first ? e1 : xptr[row + 1];
-            for (int32_t kk = kk_start; kk < kk_end; kk += G) {
-                const bool pre = first && kk == kk0;
-                const int32_t ent = pre ? ent0 : xlist[kk];
-                const int64_t e = ent >> 1;
-                const int side = ent & 1;
-                const int type = e < n_obs_e ? 0 : (e < n_obs_e + n_mot_e ? 1 : 2);
-                float po[3];
-                if (pre) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) po[c] = zo[c] + beta * qo[c];
-                    edge_apply(type, side, f0, f1, pi, po, y);
-                } else {
-                    const int64_t oo = 3 * static_cast<int64_t>(xoth[kk]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) po[c] = z[oo + c] + beta * p_old[oo + c];
-                    edge_apply(type, side, frec[2 * kk], frec[2 * kk + 1], pi, po, y);
+    for (int32_t t = t0; t < n_tiles; t += g) {
+        if (t != t0) {
+            __syncthreads();  // the previous tile is summed before ys is rewritten
+            tile_load(t);
+        }
+        const bool has_out = tid < 3 * (ti.y - ti.x);
+        mixed_slot_apply(S, beta, n_obs_e, n_mot_e, ys, tid);
+        __syncthreads();
+        float acc = 0.f;
+        if (has_out) {
+            const int32_t hi = min(ob1, ti.z + kTileBlocks);
+            for (int32_t c = max(ob0, ti.z); c < hi; ++c) acc += ys[a * kTileBlocks + (c - ti.z)];
+        }
+        // a pose with more than 256 entries (a tile of its own) continues in
+        // extra chunks (the condition is the same in the whole workgroup)
+        if (ti.w > ti.z + kTileBlocks) {
+            const int32_t ext = mt_ext[t];
+            for (int32_t cb = ti.z + kTileBlocks, h = ext; cb < ti.w; cb += kTileBlocks, ++h) {
+                MixedSlotRegs E;
+                mixed_slot_load(static_cast<int64_t>(h) * kTileBlocks + tid, mt_ent, mt_own, mt_oth,
+                                frec, z, p_old, E);
+                __syncthreads();  // the previous chunk is summed
+                mixed_slot_apply(E, beta, n_obs_e, n_mot_e, ys, tid);
+                __syncthreads();
+                if (has_out) {
+                    const int32_t hi = min(ob1, cb + kTileBlocks);
+                    for (int32_t c = max(ob0, cb); c < hi; ++c) acc += ys[a * kTileBlocks + (c - cb)];
                 }
             }
         }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            y[0] += __shfl_xor(y[0], off, G);
-            y[1] += __shfl_xor(y[1], off, G);
-            y[2] += __shfl_xor(y[2], off, G);
-        }
-        if (row < n && lane == 0) {
-            if (row == 0) {
-                y[0] += anchor * pi[0];
-                y[1] += anchor * pi[1];
-                y[2] += anchor * pi[2];
-            }
-            const int64_t o = 3 * row;
-            p_new[o] = pi[0]; p_new[o + 1] = pi[1]; p_new[o + 2] = pi[2];
-            q[o] = y[0]; q[o + 1] = y[1]; q[o + 2] = y[2];
-            pq[0] += static_cast<double>(pi[0]) * y[0] + static_cast<double>(pi[1]) * y[1] +
-                     static_cast<double>(pi[2]) * y[2];
+        if (has_out) {
+            const int32_t row = ti.x + tid / 3;
+            const float pi = zr + beta * pr;
+            if (row == 0) acc += anchor * pi;
+            const int64_t o = 3 * static_cast<int64_t>(row) + a;
+            p_new[o] = pi;
+            q[o] = acc;
+            pq[0] += static_cast<double>(pi) * acc;
         }
     }
-    write_partials<1>(pq, pq_out, kSpmvMaxGrid, bx);  // slot bx: its rows' sum
+    write_partials<1>(pq, pq_out, kSpmvMaxGrid, t0);  // slot t0: the sum of tiles t0, t0 + grid, ...
     if (blockIdx.x == 0 && threadIdx.x == 0) status->iters = k;
 }
 
@@ -543,12 +574,11 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
                        gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
-    const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);
+    const unsigned int gs = mixed_spmv_grid(g), gu = grid_for(n, 1);
     const int32_t every = o.refresh_every > 0 ? o.refresh_every : 64;
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
     double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
     PcgStatus *st = g.status;
-    const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
     GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
     hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        b, n, o.max_iter, g.dinv32, x_out, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
@@ -567,11 +597,12 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
                 is_true = 1;
             }
             float *pold = g.p32[k & 1], *pnew = g.p32[(k + 1) & 1];
-            hipLaunchKernelGGL(mixed_spmv_kernel<kLanesMixed>, dim3(gs), dim3(kThreads), 0, g.stream,
-                               g.xptr, g.xlist, g.xoth, frec, g.h.n_obs_e, g.h.n_mot_e,
-                               static_cast<float>(g.anchor), g.z32, pold, pnew, g.q32, n, k,
-                               is_true, o.tol, PR[k & 1], static_cast<int32_t>(gu),
-                               PR[(k + 1) & 1], static_cast<int32_t>(gu), PQ, st);
+            hipLaunchKernelGGL(mixed_spmv_kernel, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
+                               g.mt_ext, g.mt_ent, g.mt_own, g.mt_oth, g.frec, g.mt_tiles, g.xptr,
+                               g.h.n_obs_e, g.h.n_mot_e, static_cast<float>(g.anchor), g.z32, pold,
+                               pnew, g.q32, k, is_true, o.tol, PR[k & 1],
+                               static_cast<int32_t>(gu), PR[(k + 1) & 1],
+                               static_cast<int32_t>(gu), PQ, st);
             hipLaunchKernelGGL(mixed_update_kernel, dim3(gu), dim3(kThreads), 0, g.stream, pnew,
                                g.q32, g.dinv32, x_out, g.r32, g.z32, n, PQ,
                                static_cast<int32_t>(gs), PR[k & 1], static_cast<int32_t>(gu),
@@ -604,19 +635,18 @@ int time_mixed_spmv(DeviceGraph &g, int reps, double *us)
 {
     GSLAM_TRY(g.mixed_factors());
     const int32_t n = g.h.n;
-    const unsigned int gs = mixed_spmv_grid(n), gu = grid_for(n, 1);
+    const unsigned int gs = mixed_spmv_grid(g), gu = grid_for(n, 1);
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
     double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
-    const float4 *frec = reinterpret_cast<const float4 *>(g.frec);
     hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        g.xi, n, 1, g.dinv32, g.x, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
                        g.status);
     auto spmv = [&]() {
-        hipLaunchKernelGGL(mixed_spmv_kernel<kLanesMixed>, dim3(gs), dim3(kThreads), 0, g.stream,
-                           g.xptr, g.xlist, g.xoth, frec, g.h.n_obs_e, g.h.n_mot_e,
-                           static_cast<float>(g.anchor), g.z32, g.p32[0], g.p32[1], g.q32, n, 0, 1,
-                           1e-30, PR[0], static_cast<int32_t>(gu), PR[1], static_cast<int32_t>(gu),
-                           PQ, g.status);
+        hipLaunchKernelGGL(mixed_spmv_kernel, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
+                           g.mt_ext, g.mt_ent, g.mt_own, g.mt_oth, g.frec, g.mt_tiles, g.xptr,
+                           g.h.n_obs_e, g.h.n_mot_e, static_cast<float>(g.anchor), g.z32, g.p32[0],
+                           g.p32[1], g.q32, 0, 1, 1e-30, PR[0], static_cast<int32_t>(gu), PR[1],
+                           static_cast<int32_t>(gu), PQ, g.status);
     };
     for (int i = 0; i < 3; ++i) spmv();
     GSLAM_HIP(hipGetLastError());
