@@ -81,9 +81,11 @@ def update_bytes(n_poses):
 
 def mixed_spmv_bytes(n_poses, n_inc):
     """Algorithmic bytes of one mixed (FP32 factorized) SpMV launch: per
-    incidence entry (two per edge) 8 FP32 factors, the other pose's index and
-    the edge entry (40 B), per pose its row pointer and p_old, z read and p_new,
-    q written (FP32: 4 x 12 B)."""
+    incidence entry (two per edge) its edge entry, the other pose's index and
+    8 FP32 factors (40 B); per pose z and p_old read and p_new and q written
+    (FP32: 4 x 12 B).  The kernel's tile-padded layout also stores each
+    entry's own pose (4 B) and pads the tiles (about 1%); those bytes and the
+    row pointers are not counted."""
     return n_inc * 40 + n_poses * 48
 
 

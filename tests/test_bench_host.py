@@ -11,11 +11,14 @@ from gslam import generators as G
 
 def test_byte_counts():
     # per block 9 FP64 values + an int32 column; per row the int32 pointer,
-    # z and p_old read and p_new and q written (4 x 3 doubles)
-    assert bench.spmv_bytes(1, 1) == 76 + 8 + 96
-    assert bench.spmv_bytes(100000, 430854) == 42744908
-    # per pose x, r, p, q and D^-1 read, x, r, z written
-    assert bench.update_bytes(100000) == 24000000
+    # z, p_old and x read and p_new, q and x written (6 x 3 doubles)
+    assert bench.spmv_bytes(1, 1) == 76 + 8 + 144
+    assert bench.spmv_bytes(100000, 430854) == 47544908
+    # per pose r, q and D^-1 read, r and z written
+    assert bench.update_bytes(100000) == 16800000
+    # per incidence entry its edge, other pose and 8 FP32 factors; per pose
+    # z and p_old read and p_new and q written (4 x 3 floats)
+    assert bench.mixed_spmv_bytes(1, 2) == 2 * 40 + 48
 
 
 def test_rcm_relabel_is_a_permutation():
