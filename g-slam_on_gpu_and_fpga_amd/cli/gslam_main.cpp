@@ -267,8 +267,10 @@ int main(int argc, char *argv[])
         if (stats_json)
             std::fprintf(stderr,
                          "{\"round\": %u, \"iters\": %d, \"converged\": %d, \"rel_res\": %.6e, "
-                         "\"true_rel_res\": %.6e, \"solve_ms\": %.4f}\n",
-                         r, st.iters, st.converged, st.rel_res, st.true_rel_res, st.ms);
+                         "\"true_rel_res\": %.6e, \"solve_ms\": %.4f, \"assembly_ms\": %.4f, "
+                         "\"setup_ms\": %.4f, \"update_ms\": %.4f}\n",
+                         r, st.iters, st.converged, st.rel_res, st.true_rel_res, st.ms,
+                         st.ms_assembly, st.ms_setup, st.ms_update);
         if (gj && use_diff) {
             if (diff < 0.01) break;
             // separate plateaus for even and odd rounds (a 2-cycle of the

@@ -29,6 +29,8 @@ DeviceGraph::~DeviceGraph()
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev2) (void)hipEventDestroy(ev2);
+    for (hipEvent_t e : ev_phase)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     dense.release();
     if (vecs) (void)hipFree(vecs);
@@ -159,6 +161,8 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_HIP(hipEventCreate(&g.ev0));
     GSLAM_HIP(hipEventCreate(&g.ev1));
     GSLAM_HIP(hipEventCreate(&g.ev2));
+    GSLAM_HIP(hipEventCreate(&g.ev_phase[0]));
+    GSLAM_HIP(hipEventCreate(&g.ev_phase[1]));
     GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[0], hipEventDisableTiming));
     GSLAM_HIP(hipEventCreateWithFlags(&g.ev_poll[1], hipEventDisableTiming));
     GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.status_host), sizeof(PcgStatus)));
@@ -321,6 +325,11 @@ int gslam_set_poses(gslam_graph *gh, const double *poses)
 {
     if (!gh || !poses) return GSLAM_ERR_ARG;
     DeviceGraph &g = gh->g;
+    for (int64_t k = 0; k < g.dim(); ++k)
+        if (!std::isfinite(poses[k])) {
+            set_error("gslam_set_poses: pose value " + std::to_string(k) + " is not finite");
+            return GSLAM_ERR_ARG;
+        }
     g.assembled = g.solved = false;
     return g.upload(g.poses, poses, 3 * static_cast<int64_t>(g.h.n) * 8);
 }
@@ -345,6 +354,10 @@ int gslam_obs_cost(gslam_graph *gh, double *cost)
 int gslam_set_lambda(gslam_graph *gh, double lambda)
 {
     if (!gh) return GSLAM_ERR_ARG;
+    if (!std::isfinite(lambda)) {
+        set_error("gslam_set_lambda: lambda is not finite");
+        return GSLAM_ERR_ARG;
+    }
     gh->g.lambda = lambda;
     gh->g.assembled = gh->g.solved = false;
     return GSLAM_OK;
@@ -356,7 +369,13 @@ int gslam_set_log_weights(gslam_graph *gh, const double *log_w)
     DeviceGraph &g = gh->g;
     std::vector<double> w(static_cast<size_t>(g.h.n_bet_e), 1.0);
     if (log_w)
-        for (size_t k = 0; k < w.size(); ++k) w[k] = std::exp(log_w[k]);
+        for (size_t k = 0; k < w.size(); ++k) {
+            w[k] = std::exp(log_w[k]);
+            if (!std::isfinite(w[k])) {
+                set_error("gslam_set_log_weights: exp(log_w[" + std::to_string(k) + "]) is not finite");
+                return GSLAM_ERR_ARG;
+            }
+        }
     g.assembled = g.solved = false;
     return g.upload(g.bet_w, w.data(), g.h.n_bet_e * 8);
 }
@@ -365,8 +384,13 @@ int gslam_assemble(gslam_graph *gh)
 {
     if (!gh) return GSLAM_ERR_ARG;
     DeviceGraph &g = gh->g;
+    GSLAM_HIP(hipEventRecord(g.ev_phase[0], g.stream));
     GSLAM_TRY(gslam::launch_assembly(g));
+    GSLAM_HIP(hipEventRecord(g.ev_phase[1], g.stream));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
+    float ms = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev_phase[0], g.ev_phase[1]));
+    g.ms_assembly = ms;
     g.assembled = true;
     g.from_edges = true;
     g.factors_valid = g.frec != nullptr;
@@ -471,15 +495,33 @@ static int check_opts(const gslam_solve_opts *o)
     return GSLAM_OK;
 }
 
+// A NaN or Inf met by a solver's recurrence (flags bit 1) or left in the
+// solution (a non-finite true residual) fails the solve with
+// GSLAM_ERR_NONFINITE; the stats stay valid.
+static int nonfinite(gslam_solve_stats &s)
+{
+    if (!(s.flags & 2) && std::isfinite(s.true_rel_res)) return GSLAM_OK;
+    s.flags |= 2;
+    s.converged = 0;
+    set_error("NaN or Inf in the system, the right-hand side or the solution");
+    return GSLAM_ERR_NONFINITE;
+}
+
 static int solve_into(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x,
                       gslam_solve_stats &s)
 {
     std::memset(&s, 0, sizeof(s));
+    s.ms_assembly = g.ms_assembly;
     if (o.solver == GSLAM_SOLVER_GJ_FP64 || o.solver == GSLAM_SOLVER_GJ_MIXED ||
-        o.solver == GSLAM_SOLVER_GJ_BLOCKED64)
-        return gslam::dense_solve(g, o, b, x, s);
+        o.solver == GSLAM_SOLVER_GJ_BLOCKED64) {
+        const int rc = gslam::dense_solve(g, o, b, x, s);
+        s.ms_assembly = g.ms_assembly;
+        s.ms_setup = s.ms_factor;
+        return rc == GSLAM_OK ? nonfinite(s) : rc;
+    }
     if (o.solver == GSLAM_SOLVER_PCG_MIXED) GSLAM_TRY(g.mixed_factors());
     GSLAM_TRY(gslam::pcg_solve(g, o, b, x, s));
+    GSLAM_TRY(nonfinite(s));
     if (!s.converged && !(s.flags & 1) && o.max_iter > 0 && s.iters >= o.max_iter)
         return GSLAM_ERR_NOT_CONVERGED;
     return GSLAM_OK;
@@ -555,7 +597,16 @@ int gslam_gn_step(gslam_graph *gh, const gslam_solve_opts *opts, gslam_solve_sta
     GSLAM_TRY(gslam_assemble(gh));
     int rc = gslam_solve(gh, opts, stats);
     if (rc != GSLAM_OK && rc != GSLAM_ERR_NOT_CONVERGED) return rc;
+    DeviceGraph &g = gh->g;
+    GSLAM_HIP(hipEventRecord(g.ev_phase[0], g.stream));
     int rc2 = gslam_apply_update(gh, diff_out);
+    if (rc2 == GSLAM_OK && stats) {
+        GSLAM_HIP(hipEventRecord(g.ev_phase[1], g.stream));
+        GSLAM_HIP(hipEventSynchronize(g.ev_phase[1]));
+        float ms = 0.f;
+        GSLAM_HIP(hipEventElapsedTime(&ms, g.ev_phase[0], g.ev_phase[1]));
+        stats->ms_update = ms;
+    }
     return rc2 != GSLAM_OK ? rc2 : rc;
 }
 

@@ -1008,10 +1008,10 @@ bgj_cg_x_kernel(const double *__restrict__ p, double *__restrict__ x, int64_t n,
     if (status->done) return;
     double pq[2];
     sum_partials<2>(pq_part, np_pq, kPartStride, pq);
-    if (pq[0] == 0.0) {
+    if (pq[0] == 0.0 || !isfinite(pq[0])) {  // p.q == 0, or NaN / Inf (bit 1)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
-            status->flags |= 1;
+            status->flags |= pq[0] == 0.0 ? 1 : 2;
         }
         return;
     }

@@ -121,6 +121,22 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
         }
     }
 
+    // ---- every value the edge kernels read must be finite ----
+    auto finite = [](const double *v, int64_t n) {
+        for (int64_t k = 0; k < n; ++k)
+            if (!std::isfinite(v[k])) return false;
+        return true;
+    };
+    const double scal[9] = {d.lambda, d.anchor, d.mns[0], d.mns[1], d.mns[2], d.mns[3],
+                            d.snr[0], d.snr[1], g.n_mot_e ? d.delta : 1.0};
+    if (!finite(d.poses, 3 * static_cast<int64_t>(N)))
+        return fail(err, GSLAM_ERR_ARG, "a pose value is NaN or Inf");
+    if (!finite(scal, 9))
+        return fail(err, GSLAM_ERR_ARG, "lambda, anchor, delta, mns or snr is NaN or Inf");
+    if (!finite(g.obs_param.data(), 4 * g.n_obs_e) || !finite(g.mot_param.data(), 2 * g.n_mot_e) ||
+        !finite(g.bet_param.data(), 6 * g.n_bet_e))
+        return fail(err, GSLAM_ERR_ARG, "an observation, control or between-edge value is NaN or Inf");
+
     // ---- BSR pattern: per block row, sorted unique neighbour columns ----
     std::vector<int64_t> deg(static_cast<size_t>(N) + 1, 0);
     std::vector<uint8_t> touched(static_cast<size_t>(N), 0);

@@ -125,6 +125,8 @@ struct DeviceGraph {
     double *scal = nullptr;      // small device scalars (diff etc.)
     double *scal_host = nullptr; // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev_poll[2] = {nullptr, nullptr};
+    hipEvent_t ev_phase[2] = {nullptr, nullptr};  // around gslam_assemble / the pose update
+    double ms_assembly = 0.0;    // device time of the last gslam_assemble
 
     // factorized FP32 operator of the mixed solver, in the tiles of
     // build_mixed_tiles (graph_build.h): per tile {first pose, end pose,

@@ -304,6 +304,15 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
         return;
     }
     const double rr = red_v[0], rz = red_v[1];
+    if (!isfinite(rr) || !isfinite(rz)) {  // NaN / Inf in the residual (flag bit 1)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->done = 1;
+            status->flags |= 2;
+            status->iters = k;
+        }
+        if (apply_x) apply_x_only();
+        return;
+    }
     double tol;
     if (k == 0) {
         tol = tol_rel * sqrt(rr);
@@ -377,10 +386,12 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
     red_v[0] = strided_sum<kThreads>(pq_in, np_pq, 0.0);
     block_allsum<1, kThreads>(red_v, red);
     if (done_in) return;
-    if (red_v[0] == 0.0) {  // MatrixOps.c:357
+    // p.q == 0: the reference's break (MatrixOps.c:357); not finite: a NaN or
+    // Inf in the system or the right-hand side (flag bit 1)
+    if (red_v[0] == 0.0 || !isfinite(red_v[0])) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
-            status->flags |= 1;
+            status->flags |= red_v[0] == 0.0 ? 1 : 2;
         }
         return;
     }
@@ -592,6 +603,7 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
                        n, o.max_iter, g.dinv, static_cast<float *>(nullptr), x_out, g.r, g.z,
                        g.p[0], g.p[1], PR[0], st);
     GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev2, g.stream));  // end of the setup phase
     int32_t k = 0;
     // optional in-solve timing: events around the SpMV and the update of
     // every profile_every-th iteration (4 events per sample)
@@ -647,8 +659,10 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
     GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
     GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
-    float ms = 0.f;
+    float ms = 0.f, ms_setup = 0.f;
     GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    GSLAM_HIP(hipEventElapsedTime(&ms_setup, g.ev0, g.ev2));
+    s.ms_setup = ms_setup;
     const PcgStatus &ps = *g.status_host;
     s.iters = ps.iters;
     s.converged = ps.converged;

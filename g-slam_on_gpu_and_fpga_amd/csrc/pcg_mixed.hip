@@ -285,6 +285,14 @@ mixed_spmv_kernel(const int4 *__restrict__ mt_info, const int32_t *__restrict__ 
     block_allsum<3, kThreads>(red_v, red);
     if (done_in) return;
     const double rr = red_v[0];
+    if (!isfinite(rr)) {  // NaN / Inf in the residual (flag bit 1)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            status->done = 1;
+            status->flags |= 2;
+            status->iters = k;
+        }
+        return;
+    }
     if (k == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->bnorm = sqrt(rr);
@@ -383,10 +391,10 @@ mixed_update_kernel(const float *__restrict__ p, const float *__restrict__ q,
     red_v[1] = strided_sum<kThreads, 2>(pr_cur + kPartStride, np_cur, 0.0);
     block_allsum<2, kThreads>(red_v, red);
     if (done_in) return;
-    if (red_v[0] == 0.0) {
+    if (red_v[0] == 0.0 || !isfinite(red_v[0])) {  // p.q == 0, or NaN / Inf (bit 1)
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             status->done = 1;
-            status->flags |= 1;
+            status->flags |= red_v[0] == 0.0 ? 1 : 2;
         }
         return;
     }
@@ -541,7 +549,7 @@ int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, 
     if (handover > o.tol) o1.tol = handover;
     GSLAM_TRY(mixed_phase(g, o1, b, x_out, s));
     s.iters_fp64 = 0;
-    if (!(handover > 0.0) || s.true_rel_res < o.tol || (s.flags & 1)) return GSLAM_OK;
+    if (!(handover > 0.0) || s.true_rel_res < o.tol || (s.flags & 3)) return GSLAM_OK;
     GSLAM_TRY(g.vecs_reserve());
     const int64_t dim = g.dim();
     double *r2 = g.vecs + 2 * dim, *d = g.vecs + 3 * dim;
@@ -584,6 +592,7 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
                        b, n, o.max_iter, g.dinv32, x_out, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
                        st);
     GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev2, g.stream));  // end of the setup phase
     int32_t k = 0;
     auto refresh = [&]() {
         hipLaunchKernelGGL(mixed_refresh_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.brow_ptr,
@@ -618,8 +627,10 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
     GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
     GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
     GSLAM_HIP(hipStreamSynchronize(g.stream));
-    float ms = 0.f;
+    float ms = 0.f, ms_setup = 0.f;
     GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    GSLAM_HIP(hipEventElapsedTime(&ms_setup, g.ev0, g.ev2));
+    s.ms_setup = ms_setup;
     const PcgStatus &ps = *g.status_host;
     s.iters = ps.iters;
     s.converged = ps.converged;

@@ -24,6 +24,7 @@ LIB_PATH = os.environ.get("GSLAM_LIB", os.path.join(PKG_DIR, "libgslam.so"))
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_STATE, ERR_TOO_LARGE, ERR_NOT_CONVERGED = (
     -1, -2, -3, -4, -5, -6, -7)
+ERR_NONFINITE = -8
 SOLVERS = {"pcg64": 0, "mixed": 1, "gj64": 2, "gj_mixed": 3, "gj64_blocked": 4}
 KERNELS = {"spmv64": 0, "update64": 1, "spmv_mixed": 2, "assembly": 3, "gj_trailing": 4}
 REFINE = {"ir": 0, "cg": 1}
@@ -65,7 +66,8 @@ class _Stats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("reliable_updates", C.c_int32),
                 ("flags", C.c_int32), ("rel_res", C.c_double), ("true_rel_res", C.c_double),
                 ("ms", C.c_double), ("ms_factor", C.c_double), ("spmv_us", C.c_double),
-                ("update_us", C.c_double), ("iters_fp64", C.c_int32)]
+                ("update_us", C.c_double), ("iters_fp64", C.c_int32),
+                ("ms_assembly", C.c_double), ("ms_setup", C.c_double), ("ms_update", C.c_double)]
 
 
 _LIB = None
@@ -228,7 +230,8 @@ def _stats(s: _Stats) -> dict:
     return dict(iters=s.iters, converged=bool(s.converged), reliable_updates=s.reliable_updates,
                 flags=s.flags, rel_res=s.rel_res, true_rel_res=s.true_rel_res, ms=s.ms,
                 ms_factor=s.ms_factor, spmv_us=s.spmv_us, update_us=s.update_us,
-                iters_fp64=s.iters_fp64)
+                iters_fp64=s.iters_fp64, ms_assembly=s.ms_assembly, ms_setup=s.ms_setup,
+                ms_update=s.ms_update)
 
 
 class Graph:

@@ -34,6 +34,8 @@ extern "C" {
 #define GSLAM_ERR_STATE     -5  /* call out of order (e.g. solve before assemble) */
 #define GSLAM_ERR_TOO_LARGE -6  /* size beyond a documented limit             */
 #define GSLAM_ERR_NOT_CONVERGED -7 /* solver hit its iteration cap (stats valid) */
+#define GSLAM_ERR_NONFINITE -8 /* NaN or Inf in the system, the right-hand side
+                                  or the solution (stats valid, flags bit 1) */
 
 typedef struct gslam_graph gslam_graph;
 
@@ -141,7 +143,8 @@ typedef struct {
                                    iterations executed                      */
     int32_t converged;          /* 1 if the tolerance was met                */
     int32_t reliable_updates;   /* PCG_MIXED: FP64 residual refreshes        */
-    int32_t flags;              /* bit0: p.q == 0 break (MatrixOps.c:357)     */
+    int32_t flags;              /* bit0: p.q == 0 break (MatrixOps.c:357);
+                                   bit1: NaN / Inf met (GSLAM_ERR_NONFINITE)  */
     double rel_res;             /* final ||r||/||xi|| of the solver's residual */
     double true_rel_res;        /* ||xi - Omega dx|| / ||xi||, FP64          */
     double ms;                  /* device time of the solve (HIP events)     */
@@ -151,6 +154,12 @@ typedef struct {
     int32_t iters_fp64;         /* PCG_MIXED with mixed_handover: FP64 PCG
                                    iterations of the correction (included in
                                    iters)                                    */
+    /* Phase timers (device time, HIP events), milliseconds: */
+    double ms_assembly;         /* the handle's last gslam_assemble (edges,
+                                   Omega / xi gather)                         */
+    double ms_setup;            /* PCG: tile copy, block-Jacobi inverses,
+                                   r0 / z0 (part of ms); GJ: = ms_factor      */
+    double ms_update;           /* gslam_gn_step: pose update and diff; else 0 */
 } gslam_solve_stats;
 
 /* Fills opts with the reference defaults (PCG FP64, tol 1e-6, 500 iterations:

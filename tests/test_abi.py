@@ -76,6 +76,27 @@ def test_argument_validation_before_any_device_call():
     assert ei.value.code == gslam.ERR_ARG
 
 
+def test_non_finite_inputs_rejected_before_any_device_call():
+    """NaN / Inf in poses, edge values or model constants: GSLAM_ERR_ARG from
+    the host-side structure build, with the offending input named."""
+    nan, inf = float("nan"), float("inf")
+    bet = dict(i=[0], j=[1], meas=[1.0, 0, 0], info=[1.0, 1, 1])
+    cases = [
+        (dict(poses=[0.0, 0.0, nan, 1, 0, 0], between=bet), "pose"),
+        (dict(poses=[0.0, 0, 0, 1, 0, 0], between=dict(bet, meas=[inf, 0, 0])), "between"),
+        (dict(poses=[0.0, 0, 0, 1, 0, 0], between=dict(bet, info=[1.0, nan, 1])), "between"),
+        (dict(poses=[0.0, 0, 0, 1, 0, 0], between=bet, lam=nan), "lambda"),
+        (dict(poses=[0.0, 0, 0, 1, 0, 0], obs=dict(z_step=[0, 1], z_lm=[3, 3],
+                                                    z_range=[1.0, nan], z_bearing=[0.0, 0.0])),
+         "observation"),
+    ]
+    for kw, what in cases:
+        poses = kw.pop("poses")
+        with pytest.raises(gslam.GSlamError) as ei:
+            gslam.Graph(poses, **kw)
+        assert ei.value.code == gslam.ERR_ARG and what in str(ei.value), (what, str(ei.value))
+
+
 def test_read_log_matches_oracle_reader(oracle):
     from conftest import data_path
     import numpy as np

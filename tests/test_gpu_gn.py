@@ -8,6 +8,7 @@ set and named below.  Converged GN (tol 1e-12) on the SE(2) pose graphs is
 compared at 1e-6 m / rad, the final-pose criterion.
 """
 import ctypes as C
+import json
 import os
 import subprocess
 
@@ -216,3 +217,26 @@ def test_reference_entry_points_with_device_buffers(oracle):
     L.inv_cuda(C.c_void_p(out.data_ptr()), a.ctypes.data_as(C.c_void_p), 50)
     torch.cuda.synchronize()
     assert np.allclose(out.cpu().numpy().reshape(50, 50), np.linalg.inv(a), rtol=1e-10, atol=1e-12)
+
+
+def test_phase_timers():
+    """Device time per phase of a GN step (HIP events): assembly, solver
+    setup (part of the solve), pose update; and the CLI's JSON side channel
+    carries them."""
+    gc = G.grid_city(2000, seed=1)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        for solver in ("pcg64", "mixed", "gj64_blocked"):
+            st, _ = g.gn_step(solver, 1e-8, 100000)
+            assert st["ms_assembly"] > 0 and st["ms_update"] > 0, (solver, st)
+            assert 0 < st["ms_setup"] <= st["ms"], (solver, st)
+        g.assemble()
+        st = g.solve("pcg64", 1e-8, 100000)
+        assert st["ms_assembly"] > 0 and st["ms_update"] == 0
+    path = data_path("cityTrees100.log.txt")
+    out = subprocess.run([GSLAM_BIN, "1", "0.1", "2", "0", path], capture_output=True, text=True,
+                         env=dict(os.environ, GSLAM_STATS="1"), timeout=120, check=True)
+    recs = [json.loads(l) for l in out.stderr.splitlines() if l.startswith("{")]
+    assert len(recs) == 2
+    for r in recs:
+        assert {"solve_ms", "assembly_ms", "setup_ms", "update_ms"} <= set(r)
+        assert r["assembly_ms"] > 0 and r["update_ms"] > 0 and 0 < r["setup_ms"] <= r["solve_ms"]

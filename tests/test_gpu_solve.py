@@ -17,6 +17,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 from conftest import GOLDEN_SETS, data_path, golden
+from refsolve import refined_direct
 
 import gslam
 from gslam import generators as G
@@ -126,6 +127,36 @@ def test_mixed_certifies_true_residual(name):
         assert e2 <= 2e-9
 
 
+def test_mixed_pose_with_more_entries_than_a_tile():
+    """A hub pose with 700 between edges: its row of the FP32 factorized
+    operator spans three 256-entry chunks (a tile of its own, summed chunk by
+    chunk); the solve still meets its FP64 true residual and the direct
+    solution."""
+    rng = np.random.default_rng(4)
+    n = 800
+    poses = np.column_stack([rng.uniform(-20, 20, n), rng.uniform(-20, 20, n),
+                             rng.uniform(-np.pi, np.pi, n)]).ravel()
+    bi = np.r_[np.zeros(700, np.int32), np.arange(1, n - 1, dtype=np.int32)]
+    bj = np.r_[np.arange(1, 701, dtype=np.int32), np.arange(2, n, dtype=np.int32)]
+    meas = rng.normal(size=3 * len(bi)) * 0.1
+    info = np.tile([100.0, 100.0, 400.0], len(bi))
+    with gslam.Graph(poses, between=dict(i=bi, j=bj, meas=meas, info=info), lam=1.0) as g:
+        g.assemble()
+        xs = refined_direct(*g.system_csr())
+        err, its = {}, {}
+        for name, solver, kw in (("pcg64", "pcg64", {}), ("mixed", "mixed", {}),
+                                 ("handover", "mixed", dict(mixed_handover=1e-4))):
+            st = g.solve(solver, 1e-11, 200000, **kw)
+            assert st["converged"] and st["true_rel_res"] < 1e-11, (name, st)
+            err[name], its[name] = rel(g.dx(), xs), st["iters"]
+        print("hub graph: rel dx err", err, "iterations", its)
+        # measured 8.6e-11 / 3.9e-8 / 1.5e-10 (1431 / 2944 / 2082 iterations); hubs of
+        # 200-256 entries (one chunk) give 1.1-1.3e-8 for mixed at the same ratio of
+        # iterations, 257-300 entries 2.3-4.9e-8
+        assert err["pcg64"] <= 1e-9 and err["mixed"] <= 2e-7 and err["handover"] <= 1e-9, err
+        assert its["mixed"] <= 3 * its["pcg64"], its
+
+
 def test_mixed_on_reference_log_model():
     """The factorized FP32 operator covers landmark-pair and motion edges."""
     with gslam.Graph.from_log(data_path("victoria-park-100.log.txt"), lam=0.1) as g:
@@ -169,6 +200,37 @@ def test_solver_buffers_allocated_on_first_use():
         sa2 = a.solve("mixed", 1e-10, 200000)
         assert a.device_bytes() == m2
         assert sa2["iters"] == sa["iters"] and np.array_equal(a.dx(), b.dx())
+
+
+def test_non_finite_system_fails_loudly():
+    """A NaN in Omega or in the right-hand side stops every solver within an
+    iteration or two with GSLAM_ERR_NONFINITE (flags bit 1), instead of running
+    to the iteration cap or returning NaN poses; non-finite poses are refused."""
+    gc = G.grid_city(400, seed=5)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        bad_rhs = xi.copy()
+        bad_rhs[7] = np.nan
+        for solver in ("pcg64", "mixed"):
+            with pytest.raises(gslam.GSlamError) as ei:
+                g.solve_rhs(bad_rhs, solver, 1e-10, 100000)
+            assert ei.value.code == gslam.ERR_NONFINITE, solver
+        # the handle stays usable
+        st = g.solve("pcg64", 1e-10, 100000)
+        assert st["converged"]
+        va_bad = va.copy()
+        va_bad[rp[30]] = np.inf
+        g.set_system_csr(va_bad, xi)
+        for solver in ("pcg64", "gj64", "gj64_blocked", "gj_mixed"):
+            with pytest.raises(gslam.GSlamError) as ei:
+                g.solve(solver, 1e-10, 100000)
+            assert ei.value.code == gslam.ERR_NONFINITE, solver
+        p = gc["poses"].copy()
+        p[5] = np.nan
+        with pytest.raises(gslam.GSlamError) as ei:
+            g.set_poses(p)
+        assert ei.value.code == gslam.ERR_ARG
 
 
 def test_zero_rhs_breaks_on_pq_zero_and_cap():
