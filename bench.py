@@ -35,6 +35,8 @@ Also reported:
   4800, 15000; the plain FP64 Gauss-Jordan up to N=2000);
 * park: the park graph (laps route) of 7,120 poses, lambda 1.0 -- FP64 and
   mixed PCG at the same tolerance against the same kind of direct solve;
+* grid10k: the FP64 PCG GN run on grid-city N=10,000 (tol 1e-10, the
+  reference stop rule within 20 rounds), with the device time per phase;
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -450,6 +452,39 @@ def park_section(args, gslam, G, np):
     return out
 
 
+def grid10k_section(args, gslam, G, np):
+    """C2: FP64 PCG GN run on the N=10,000 grid city (lambda 0.1, PCG tol
+    1e-10, the reference stop rule diff < 0.01 within 20 rounds); the poses
+    of the same run are checked against the CPU oracle in
+    tests/test_gpu_gn.py.  Whole-run and per-step milliseconds with the
+    device time per phase."""
+    gc = G.grid_city(args.grid10k_n, seed=0)
+    out = dict(n_poses=args.grid10k_n, lambda_=0.1, tol=1e-10)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
+        g.gn_step("pcg64", 1e-10, 200000)          # warm-up round (allocations, code load)
+        g.set_poses(gc["poses"])
+        hip_sync()
+        t = time.perf_counter()
+        its, diffs, phases = [], [], []
+        for _ in range(20):
+            st, diff = g.gn_step("pcg64", 1e-10, 200000)
+            its.append(st["iters"])
+            diffs.append(diff)
+            phases.append((st["ms_assembly"], st["ms_setup"], st["ms"], st["ms_update"]))
+            if diff < 0.01:
+                break
+        hip_sync()
+        ms = 1000.0 * (time.perf_counter() - t)
+    ph = np.mean(np.array(phases), axis=0)
+    out.update(rounds=len(its), ms_run=ms, ms_per_step=ms / len(its), pcg_iters=int(sum(its)),
+               us_per_iter=1000.0 * float(np.sum(np.array(phases)[:, 2])) / max(sum(its), 1),
+               last_diff=diffs[-1], phase_ms_mean=dict(assembly=ph[0], setup=ph[1], solve=ph[2],
+                                                        update=ph[3]))
+    log(f"grid10k pcg64: {len(its)} GN rounds in {ms:.1f} ms ({ms / len(its):.1f} ms per step, "
+        f"{sum(its)} PCG its, last diff {diffs[-1]:.2e})")
+    return out
+
+
 def gj_cpu_baseline(args, np):
     """The reference's FP64 Gauss-Jordan inv() (src/cpu_pcg_fp64/MatrixOps.c:29-91,
     oracle/_ref) on the host, one thread, at n = args.gj_cpu_n on the N=800
@@ -596,6 +631,8 @@ def main():
                     help="mixed: FP64 true relative residual at which FP64 PCG takes over the "
                          "correction (0: FP32 iterations all the way)")
     ap.add_argument("--park-n", type=int, default=7120, help="park section size (0 skips)")
+    ap.add_argument("--grid10k-n", type=int, default=10000,
+                    help="size of the FP64 PCG GN-run section (0 skips)")
     ap.add_argument("--park-tol", type=float, default=1e-9)
     ap.add_argument("--park-handover", type=float, default=1e-2)
     ap.add_argument("--gj-cpu-n", type=int, default=1200,
@@ -639,6 +676,7 @@ def main():
     cpu = cpu_section(args, gc, init, info, results)
     gj = gj_section(args, gslam, G, np) if args.gj_n else None
     park = park_section(args, gslam, G, np) if args.park_n > 0 else None
+    grid10k = grid10k_section(args, gslam, G, np) if args.grid10k_n > 0 else None
     gj_cpu = gj_cpu_baseline(args, np)
     if cpu is not None and gj_cpu is not None:
         cpu["gj_inv"] = gj_cpu
@@ -675,6 +713,7 @@ def main():
         "cpu_baseline": cpu,
         "gj": gj,
         "park": park,
+        "grid10k": grid10k,
         "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
