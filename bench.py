@@ -37,6 +37,8 @@ Also reported:
   mixed PCG at the same tolerance against the same kind of direct solve;
 * grid10k: the FP64 PCG GN run on grid-city N=10,000 (tol 1e-10, the
   reference stop rule within 20 rounds), with the device time per phase;
+* large: FP64 and mixed PCG on grid-city N=1,200,000 (HBM-bound) to a true
+  relative residual of 1e-5;
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -452,6 +454,27 @@ def park_section(args, gslam, G, np):
     return out
 
 
+def large_section(args, gslam, G, np):
+    """Mixed vs FP64 PCG where both stream from HBM: grid-city N = args.large_n
+    (1.2M: 1.4 GB of BSR blocks), one solve to a true relative residual of
+    args.large_tol from the dead-reckoned guess's system.  There the FP32
+    iterations are about half the cost of FP64 ones and, to this tolerance,
+    just as many."""
+    gc = G.grid_city(args.large_n, seed=2)
+    out = dict(n_poses=args.large_n, n_edges=len(gc["i"]), lambda_=0.1, tol=args.large_tol)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
+        g.assemble()
+        for name in ("pcg64", "mixed"):
+            st = g.solve(name, args.large_tol, 200000)
+            out[name] = dict(solve_ms=st["ms"], pcg_iters=st["iters"], converged=st["converged"],
+                             true_rel_res=st["true_rel_res"],
+                             us_per_iter=1000.0 * st["ms"] / max(st["iters"], 1))
+            log(f"large N={args.large_n} {name}: {st['ms']:.0f} ms, {st['iters']} its, true rel res "
+                f"{st['true_rel_res']:.2e}")
+    out["mixed_speedup_vs_fp64"] = out["pcg64"]["solve_ms"] / out["mixed"]["solve_ms"]
+    return out
+
+
 def grid10k_section(args, gslam, G, np):
     """C2: FP64 PCG GN run on the N=10,000 grid city (lambda 0.1, PCG tol
     1e-10, the reference stop rule diff < 0.01 within 20 rounds); the poses
@@ -631,6 +654,9 @@ def main():
                     help="mixed: FP64 true relative residual at which FP64 PCG takes over the "
                          "correction (0: FP32 iterations all the way)")
     ap.add_argument("--park-n", type=int, default=7120, help="park section size (0 skips)")
+    ap.add_argument("--large-n", type=int, default=1200000,
+                    help="size of the HBM-regime mixed vs FP64 section (0 skips)")
+    ap.add_argument("--large-tol", type=float, default=1e-5)
     ap.add_argument("--grid10k-n", type=int, default=10000,
                     help="size of the FP64 PCG GN-run section (0 skips)")
     ap.add_argument("--park-tol", type=float, default=1e-9)
@@ -677,6 +703,7 @@ def main():
     gj = gj_section(args, gslam, G, np) if args.gj_n else None
     park = park_section(args, gslam, G, np) if args.park_n > 0 else None
     grid10k = grid10k_section(args, gslam, G, np) if args.grid10k_n > 0 else None
+    large = large_section(args, gslam, G, np) if args.large_n > 0 else None
     gj_cpu = gj_cpu_baseline(args, np)
     if cpu is not None and gj_cpu is not None:
         cpu["gj_inv"] = gj_cpu
@@ -714,6 +741,7 @@ def main():
         "gj": gj,
         "park": park,
         "grid10k": grid10k,
+        "large": large,
         "adjoint": adj,
     }
     print(json.dumps(out), flush=True)

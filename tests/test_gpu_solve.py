@@ -157,6 +157,23 @@ def test_mixed_pose_with_more_entries_than_a_tile():
         assert its["mixed"] <= 3 * its["pcg64"], its
 
 
+def test_mixed_many_tiles_per_workgroup():
+    """Grid city N=400,000: 5,000+ operator tiles, more than the mixed SpMV's
+    4,096 workgroups, so workgroups walk several tiles (round robin).  To a
+    true FP64 residual of 1e-4 / 1e-5 the FP32 iterations follow the FP64
+    ones almost step for step (measured 6,272 vs 6,226 and 23,680 vs 23,630
+    iterations); a tile skipped or summed twice would leave the FP64 true
+    residual stuck."""
+    gc = G.grid_city(400000, seed=2)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
+        g.assemble()
+        for tol in (1e-4, 1e-5):
+            a = g.solve("pcg64", tol, 100000)
+            b = g.solve("mixed", tol, 100000)
+            assert a["converged"] and b["converged"] and b["true_rel_res"] < tol, (tol, a, b)
+            assert abs(b["iters"] - a["iters"]) <= 0.02 * a["iters"], (tol, a["iters"], b["iters"])
+
+
 def test_mixed_on_reference_log_model():
     """The factorized FP32 operator covers landmark-pair and motion edges."""
     with gslam.Graph.from_log(data_path("victoria-park-100.log.txt"), lam=0.1) as g:
