@@ -93,6 +93,18 @@ def mixed_spmv_bytes(n_poses, n_inc):
     return n_inc * 40 + n_poses * 48
 
 
+def assembly_bytes(n_poses, n_edges, n_blocks):
+    """Algorithmic bytes of one assembly of a between-edge graph (edge, block
+    and xi kernels): per edge its two poses, 7 parameters and the 48-double
+    staged record written (48 + 56 + 384 B); per block its contribution
+    pointer and values written (4 + 72 B) and its contributions (4 per edge:
+    index and 9 staged values, 4 + 72 B each); per pose its xi pointer and 3
+    xi values written (4 + 24 B) and its 2 contributions per incident edge
+    end (index and 3 staged values, 4 + 24 B each)."""
+    return (n_edges * (48 + 56 + 384) + n_blocks * 76 + 4 * n_edges * 76 + n_poses * 28
+            + 2 * n_edges * 28)
+
+
 def measured_traffic():
     """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
@@ -616,6 +628,17 @@ def profile_kernel(args, gslam, G):
             log(f"spmv_mixed N={args.n}: {us:.2f} us/launch (23 launches), {byts / 1e6:.1f} MB "
                 f"algorithmic, {gbs:.0f} GB/s = {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
         return
+    if args.profile_kernel == "assembly":
+        gc = G.grid_city(args.n, seed=args.seed)
+        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
+            g.assemble()
+            us = g.time_kernel("assembly", 20)
+            byts = assembly_bytes(g.info["n_poses"], g.info["n_edges"], g.info["n_blocks"])
+            log(f"assembly N={args.n}: {us:.2f} us per assembly (edge, block and xi kernels, 23 "
+                f"launches each), {byts / 1e6:.1f} MB algorithmic, "
+                f"{byts / (us * 1e-6) / 1e9:.0f} GB/s = "
+                f"{100 * byts / (us * 1e-6) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak")
+        return
     if args.profile_kernel in ("spmv64", "update64"):
         k = args.profile_kernel
         gc = G.grid_city(args.n, seed=args.seed)
@@ -681,7 +704,7 @@ def main():
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
     ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
     ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "spmv_mixed", "gj_trailing",
-                                                 "copy", "stream"],
+                                                 "assembly", "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "
                          "passes; 'copy' is a 64 MiB device copy for calibrating the counters, "
                          "'stream' times a 2 GiB device copy: the achievable HBM bandwidth)")
