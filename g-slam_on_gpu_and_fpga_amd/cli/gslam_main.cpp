@@ -153,6 +153,22 @@ int env_int(const char *name, int dflt)
 
 }  // namespace
 
+// The poses as "%u %.17g %.17g %.17g" lines (x_step, x, y, theta), the
+// POSE_DUMP_FILE format (GSLAM.c:198-213); written to a temporary file and
+// renamed, so that a checkpoint is never seen half-written.
+static void dump_poses(gslam_graph *g, const LogData &d, int n, const char *path)
+{
+    std::vector<double> poses(3 * static_cast<size_t>(n));
+    if (gslam_get_poses(g, poses.data()) != GSLAM_OK) return;
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *df = std::fopen(tmp.c_str(), "w");
+    if (!df) return;
+    for (int i = 0; i < n; ++i)
+        std::fprintf(df, "%u %.17g %.17g %.17g\n", d.x_step[i], poses[3 * i], poses[3 * i + 1],
+                     poses[3 * i + 2]);
+    if (std::fclose(df) == 0) std::rename(tmp.c_str(), path);
+}
+
 int main(int argc, char *argv[])
 {
     if (argc == 6) {
@@ -235,6 +251,11 @@ int main(int argc, char *argv[])
     opts.max_iter = env_int("GSLAM_MAX_ITER", opts.max_iter);
     opts.mixed_handover = env_double("GSLAM_HANDOVER", opts.mixed_handover);
     const bool stats_json = env_int("GSLAM_STATS", 0) != 0;
+    // checkpoints: the poses after every k-th round into POSE_DUMP_FILE (the
+    // final dump overwrites the last one); rewriting the x records of the
+    // input with them resumes the run (gslam.resume_log)
+    const int dump_every = env_int("GSLAM_DUMP_EVERY", 0);
+    const char *dump_path = std::getenv("POSE_DUMP_FILE");
 
     const bool gj = opts.solver == GSLAM_SOLVER_GJ_FP64 || opts.solver == GSLAM_SOLVER_GJ_MIXED ||
                     opts.solver == GSLAM_SOLVER_GJ_BLOCKED64;
@@ -271,6 +292,8 @@ int main(int argc, char *argv[])
                          "\"setup_ms\": %.4f, \"update_ms\": %.4f}\n",
                          r, st.iters, st.converged, st.rel_res, st.true_rel_res, st.ms,
                          st.ms_assembly, st.ms_setup, st.ms_update);
+        if (dump_every > 0 && dump_path && r % static_cast<unsigned int>(dump_every) == 0)
+            dump_poses(g, d, n, dump_path);
         if (gj && use_diff) {
             if (diff < 0.01) break;
             // separate plateaus for even and odd rounds (a 2-cycle of the
@@ -301,17 +324,7 @@ int main(int argc, char *argv[])
     }
     if (draw >= 1 || gj) {
         const char *dump = std::getenv("POSE_DUMP_FILE");
-        if (dump) {
-            std::vector<double> poses(3 * static_cast<size_t>(n));
-            gslam_get_poses(g, poses.data());
-            FILE *df = std::fopen(dump, "w");
-            if (df) {
-                for (int i = 0; i < n; ++i)
-                    std::fprintf(df, "%u %.17g %.17g %.17g\n", d.x_step[i], poses[3 * i],
-                                 poses[3 * i + 1], poses[3 * i + 2]);
-                std::fclose(df);
-            }
-        }
+        if (dump) dump_poses(g, d, n, dump);
     }
     gslam_destroy(g);
     return 0;

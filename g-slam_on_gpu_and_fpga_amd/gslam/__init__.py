@@ -210,6 +210,35 @@ def read_log(path: str) -> dict:
                 n_poses=len(x))
 
 
+def resume_log(log_path: str, dump_path: str, out_path: str) -> int:
+    """Checkpoint resume: writes `log_path` to `out_path` with the pose values
+    of its x records replaced by those of a pose dump (POSE_DUMP_FILE lines
+    `step x y theta`, matched to the x records by step), every other record
+    unchanged.  The GN loop started on `out_path` continues from the dumped
+    poses.  Returns the number of x records rewritten."""
+    dumped = {}
+    with open(dump_path) as f:
+        for ln, line in enumerate(f, 1):
+            tok = line.split()
+            if len(tok) != 4:
+                raise ValueError(f"{dump_path}:{ln}: expected 'step x y theta'")
+            dumped[int(tok[0])] = tok[1:]
+    n = 0
+    with open(log_path) as f, open(out_path, "w") as g:
+        for line in f:
+            tok = [t for t in line.split(" ") if t != ""]
+            if tok and tok[0] == "x":
+                step = int(float(tok[1]))
+                if step not in dumped:
+                    raise ValueError(f"{dump_path}: no pose for x record {step}")
+                line = "x %s %s\n" % (tok[1].strip(), " ".join(dumped[step]))
+                n += 1
+            g.write(line)
+    if n != len(dumped):
+        raise ValueError(f"{dump_path}: {len(dumped)} poses for {n} x records")
+    return n
+
+
 def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
               check_every=0, refine_mode="ir", profile_every=0, mixed_handover=0.0) -> _Opts:
     o = _Opts()

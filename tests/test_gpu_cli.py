@@ -134,3 +134,20 @@ def test_between_sidecar_pose_graph(oracle, tmp_path, solver):
     d[:, 2] = (d[:, 2] + np.pi) % (2 * np.pi) - np.pi
     assert np.abs(d).max() <= 1e-6
     assert diffs[-1] < 0.01 and all(x >= 0.01 for x in diffs[:-1])
+
+
+def test_checkpoint_resume_continues_the_run(tmp_path):
+    """GSLAM_DUMP_EVERY writes the poses after every k-th round; the x records
+    rewritten with a checkpoint (gslam.resume_log) start a run that continues
+    the original one: rounds 4-6 of one 6-round run are, bit for bit, rounds
+    1-3 of a run resumed from the round-3 checkpoint."""
+    path = data_path("cityTrees100.log.txt")
+    full, ck, fin = (str(tmp_path / n) for n in ("full.txt", "ck.txt", "fin.txt"))
+    a = _gslam([1, 0.1, 6, 1, path], env=dict(POSE_DUMP_FILE=full))
+    b1 = _gslam([1, 0.1, 3, 1, path], env=dict(POSE_DUMP_FILE=ck, GSLAM_DUMP_EVERY="3"))
+    assert _diffs(b1) == _diffs(a)[:3] and len(_diffs(a)) == 6
+    resumed = str(tmp_path / "resumed.log.txt")
+    assert gslam.resume_log(path, ck, resumed) == 100
+    b2 = _gslam([1, 0.1, 3, 1, resumed], env=dict(POSE_DUMP_FILE=fin))
+    assert _diffs(b2) == _diffs(a)[3:]
+    assert np.array_equal(np.loadtxt(fin), np.loadtxt(full))

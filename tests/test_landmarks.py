@@ -39,3 +39,25 @@ def test_closed_form_and_order():
     assert np.allclose(m[1], [1.0 + np.cos(0.5), 2.0 + np.sin(0.5)])
     ids, m = gslam.landmark_means(poses, [], [], [], [])
     assert ids.size == 0 and m.shape == (0, 2)
+
+
+def test_resume_log_rewrites_only_the_x_records(tmp_path):
+    path = data_path("cityTrees100.log.txt")
+    d = gslam.read_log(path)
+    rng = np.random.default_rng(1)
+    p = d["poses"].reshape(-1, 3) + rng.normal(size=(d["n_poses"], 3))
+    dump = tmp_path / "dump.txt"
+    with open(dump, "w") as f:
+        for s, row in zip(d["x_step"], p):
+            f.write("%d %.17g %.17g %.17g\n" % (s, *row))
+    out = str(tmp_path / "out.log.txt")
+    assert gslam.resume_log(path, str(dump), out) == d["n_poses"]
+    e = gslam.read_log(out)
+    assert np.array_equal(e["poses"], p.ravel())
+    for k in ("u_nu", "u_omega", "z_step", "z_lm", "z_range", "z_bearing", "x_step"):
+        assert np.array_equal(e[k], d[k]), k
+    assert e["delta"] == d["delta"]
+    with open(dump, "a") as f:
+        f.write("999 0 0 0\n")
+    with pytest.raises(ValueError):
+        gslam.resume_log(path, str(dump), out)
