@@ -112,6 +112,7 @@ int DeviceGraph::mixed_factors()
         // padding slots keep zero factors
         GSLAM_HIP(hipMemsetAsync(f, 0, static_cast<size_t>(8 * slots) * 4, stream));
         mt_tiles = m.n_tiles;
+        mt_slots = slots;
         frec = f;  // last: frec != nullptr means every mixed buffer exists
     }
     GSLAM_TRY(launch_assembly(*this));

@@ -141,6 +141,7 @@ struct DeviceGraph {
     int32_t *mt_ext = nullptr, *mt_ent = nullptr, *mt_own = nullptr, *mt_oth = nullptr;
     int32_t *mt_pos = nullptr;
     int32_t mt_tiles = 0;
+    int64_t mt_slots = 0;        // slots of all tiles and extra chunks
     float *frec = nullptr;
 
     // Vector work space, allocated on first use and kept: 4 x 3N + n_between

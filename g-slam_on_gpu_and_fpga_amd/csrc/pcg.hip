@@ -143,6 +143,7 @@ struct SpmvTileRegs {
     double yz[3], yp[3], zo, po, xo;
 };
 
+template <bool kNt>
 __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_info,
                                       const int32_t *__restrict__ brow_ptr,
                                       const int32_t *__restrict__ tcol,
@@ -158,11 +159,11 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     // p_old (from a row clamped into the tile: unconditional, so that no
     // branch waits on them); then the gathered z / p_old and the products
     const int64_t slot = static_cast<int64_t>(t) * kTileBlocks + threadIdx.x;
-    const int64_t j = 3 * static_cast<int64_t>(tcol[slot]);
+    const int64_t j = 3 * static_cast<int64_t>(stream_load<kNt>(tcol + slot));
     const double *tv = tval + 9 * static_cast<int64_t>(t) * kTileBlocks + threadIdx.x;
     double v[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = tv[k * kTileBlocks];
+    for (int k = 0; k < 9; ++k) v[k] = stream_load<kNt>(tv + k * kTileBlocks);
     const int4 ti = tile_info[t];
     T.r0 = ti.x;
     T.r1 = ti.y;
@@ -243,7 +244,9 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
 
 // (at most 72 VGPRs: 7 workgroups per CU, 1792 on the chip, keep the 1697
 // tiles of the N = 100k grid city in one round; at 92 VGPRs, 5 per CU, the
-// kernel measured 16 us in a solve.  A 64-VGPR cap spills.)
+// kernel measured 16 us in a solve.  A 64-VGPR cap spills.)  kNt: the tile
+// copy is loaded nontemporally (it exceeds the Infinity Cache).
+template <bool kNt>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
                 const double *__restrict__ bval, const int4 *__restrict__ tile_info,
@@ -285,7 +288,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
     const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
-    if (t0 < n_tiles) spmv_tile_load(t0, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
+    if (t0 < n_tiles) spmv_tile_load<kNt>(t0, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
     const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     double red_v[2];
     red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
@@ -340,7 +343,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
         if (t != t0) {
             __syncthreads();  // the previous tile is summed before ys is rewritten
-            spmv_tile_load(t, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
+            spmv_tile_load<kNt>(t, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
         }
         pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, x, apply_x, alpha_prev, ys);
     }
@@ -355,7 +358,9 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
 // same partials (the same bits as summing them again here, without the
 // loads), x/r/z update, r.r / r.z partials.  One thread per pose: its 21
 // loads (x, r, p, q, D^-1) are issued before the p.q partials are loaded and
-// reduced; D^-1 is stored column-major per entry ([c][3i+a]).
+// reduced; D^-1 is stored column-major per entry ([c][3i+a]).  kNt: q and
+// D^-1 are loaded nontemporally (D^-1 exceeds the Infinity Cache).
+template <bool kNt>
 __global__ void __launch_bounds__(kThreads)
 pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
                   double *__restrict__ r, double *__restrict__ z, int32_t n, int32_t k,
@@ -372,12 +377,12 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             rp[c] = r[o + c];
-            qp[c] = q[o + c];
+            qp[c] = stream_load<kNt>(q + o + c);
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) di[3 * c + a] = dinv[c * dim + o + a];
+            for (int a = 0; a < 3; ++a) di[3 * c + a] = stream_load<kNt>(dinv + c * dim + o + a);
     };
     const int32_t done_in = status->done;  // set by earlier launches only
     if (i < n) load(i);
@@ -532,6 +537,24 @@ diff_kernel(const double *__restrict__ part, int32_t np, int32_t n, double *__re
     if (threadIdx.x == 0) *out = sqrt(t[0]) / sqrt(3.0 * static_cast<double>(n));
 }
 
+// The SpMV and update kernels of this graph: nontemporal variants where the
+// tile copy (76 B per slot) or q and D^-1 (96 B per pose) exceed the
+// Infinity Cache.
+using SpmvKernel = decltype(&pcg_spmv_kernel<false>);
+using UpdateKernel = decltype(&pcg_update_kernel<false>);
+
+static SpmvKernel spmv_kernel_for(const DeviceGraph &g)
+{
+    return streams_past_cache(76 * int64_t{kTileBlocks} * g.n_tiles) ? pcg_spmv_kernel<true>
+                                                                       : pcg_spmv_kernel<false>;
+}
+
+static UpdateKernel update_kernel_for(const DeviceGraph &g)
+{
+    return streams_past_cache(96 * static_cast<int64_t>(g.h.n)) ? pcg_update_kernel<true>
+                                                                : pcg_update_kernel<false>;
+}
+
 // The padded tile copy of the current block values (before each FP64 solve).
 static int launch_tile_pack(DeviceGraph &g)
 {
@@ -614,19 +637,21 @@ int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, d
         g.prof_ev.resize(4 * kMaxSamples, nullptr);
         for (hipEvent_t &e : g.prof_ev) GSLAM_HIP(hipEventCreate(&e));
     }
+    const SpmvKernel spmv_kern = spmv_kernel_for(g);
+    const UpdateKernel update_kern = update_kernel_for(g);
     auto enqueue_iters = [&](int32_t m) {
         for (int32_t c = 0; c < m; ++c, ++k) {
             double *pold = g.p[k & 1], *pnew = g.p[(k + 1) & 1];
             const bool samp = o.profile_every > 0 && k % o.profile_every == 0 && n_samp < kMaxSamples;
             hipEvent_t *ev = samp ? &g.prof_ev[4 * n_samp++] : nullptr;
             auto spmv_args = [&](auto launch) {
-                launch(pcg_spmv_kernel, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval,
+                launch(spmv_kern, g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval,
                        g.n_tiles, g.z,
                        pold, pnew, g.q, k, o.tol, PR[k & 1], static_cast<int32_t>(ge),
                        g.part + kPartOffRz, PQ, x_out, g.part + kPartOffAlpha, st);
             };
             auto update_args = [&](auto launch) {
-                launch(pcg_update_kernel, g.q, g.dinv, g.r, g.z, n, k, PQ,
+                launch(update_kern, g.q, g.dinv, g.r, g.z, n, k, PQ,
                        static_cast<int32_t>(gt), g.part + kPartOffRz + (k & 1),
                        PR[(k + 1) & 1], g.part + kPartOffAlpha + (k & 1), st);
             };
@@ -729,15 +754,17 @@ int time_kernel(DeviceGraph &g, int which, int reps, double *us)
     double *PQ = g.part + kPartOffPQT;
     if (which == GSLAM_KERNEL_SPMV_MIXED) return time_mixed_spmv(g, reps, us);
     if (which < GSLAM_KERNEL_SPMV_FP64 || which > GSLAM_KERNEL_ASSEMBLY) return GSLAM_ERR_ARG;
+    const SpmvKernel spmv_kern = spmv_kernel_for(g);
+    const UpdateKernel update_kern = update_kernel_for(g);
     auto spmv = [&]() {
-        hipLaunchKernelGGL(pcg_spmv_kernel, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
+        hipLaunchKernelGGL(spmv_kern, dim3(gt), dim3(kThreads), 0, g.stream, g.brow_ptr,
                            g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.n_tiles, g.z, g.p[0],
                            g.p[1], g.q, 0,
                            1e-30, PR[0], static_cast<int32_t>(ge), g.part + kPartOffRz, PQ,
                            g.x, g.part + kPartOffAlpha, g.status);
     };
     auto update = [&]() {
-        hipLaunchKernelGGL(pcg_update_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.q,
+        hipLaunchKernelGGL(update_kern, dim3(ge), dim3(kThreads), 0, g.stream, g.q,
                            g.dinv, g.r, g.z, n, 0, PQ, static_cast<int32_t>(gt),
                            g.part + kPartOffRz, PR[1], g.part + kPartOffAlpha, g.status);
     };

@@ -182,19 +182,22 @@ struct MixedSlotRegs {
     float zi[3], qi[3], zo[3], qo[3];
 };
 
+template <bool kNt>
 __device__ inline void mixed_slot_load(int64_t slot, const int32_t *__restrict__ mt_ent,
                                        const int32_t *__restrict__ mt_own,
                                        const int32_t *__restrict__ mt_oth,
                                        const float *__restrict__ frec, const float *__restrict__ z,
                                        const float *__restrict__ p_old, MixedSlotRegs &S)
 {
-    const int64_t oi = 3 * static_cast<int64_t>(mt_own[slot]);
-    const int64_t oo = 3 * static_cast<int64_t>(mt_oth[slot]);
-    S.ent = mt_ent[slot];
+    const int64_t oi = 3 * static_cast<int64_t>(stream_load<kNt>(mt_own + slot));
+    const int64_t oo = 3 * static_cast<int64_t>(stream_load<kNt>(mt_oth + slot));
+    S.ent = stream_load<kNt>(mt_ent + slot);
     const float *f = frec + (slot / kTileBlocks) * 8 * kTileBlocks + slot % kTileBlocks;
-    S.f0 = make_float4(f[0], f[kTileBlocks], f[2 * kTileBlocks], f[3 * kTileBlocks]);
-    S.f1 = make_float4(f[4 * kTileBlocks], f[5 * kTileBlocks], f[6 * kTileBlocks],
-                       f[7 * kTileBlocks]);
+    float fv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) fv[c] = stream_load<kNt>(f + c * kTileBlocks);
+    S.f0 = make_float4(fv[0], fv[1], fv[2], fv[3]);
+    S.f1 = make_float4(fv[4], fv[5], fv[6], fv[7]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         S.zi[c] = z[oi + c];
@@ -226,6 +229,7 @@ __device__ inline void mixed_slot_apply(const MixedSlotRegs &S, float beta, int6
     for (int a = 0; a < 3; ++a) ys[a * kTileBlocks + col] = y[a];
 }
 
+template <bool kNt>
 __global__ void __launch_bounds__(kThreads)
 mixed_spmv_kernel(const int4 *__restrict__ mt_info, const int32_t *__restrict__ mt_ext,
                   const int32_t *__restrict__ mt_ent, const int32_t *__restrict__ mt_own,
@@ -265,7 +269,7 @@ mixed_spmv_kernel(const int4 *__restrict__ mt_info, const int32_t *__restrict__ 
     int32_t ob0 = 0, ob1 = 0;
     float zr = 0.f, pr = 0.f;
     auto tile_load = [&](int32_t t) {
-        mixed_slot_load(static_cast<int64_t>(t) * kTileBlocks + tid, mt_ent, mt_own, mt_oth, frec,
+        mixed_slot_load<kNt>(static_cast<int64_t>(t) * kTileBlocks + tid, mt_ent, mt_own, mt_oth, frec,
                         z, p_old, S);
         ti = mt_info[t];
         const int32_t orow = min(ti.x + tid / 3, ti.y - 1);
@@ -333,7 +337,7 @@ mixed_spmv_kernel(const int4 *__restrict__ mt_info, const int32_t *__restrict__ 
             const int32_t ext = mt_ext[t];
             for (int32_t cb = ti.z + kTileBlocks, h = ext; cb < ti.w; cb += kTileBlocks, ++h) {
                 MixedSlotRegs E;
-                mixed_slot_load(static_cast<int64_t>(h) * kTileBlocks + tid, mt_ent, mt_own, mt_oth,
+                mixed_slot_load<kNt>(static_cast<int64_t>(h) * kTileBlocks + tid, mt_ent, mt_own, mt_oth,
                                 frec, z, p_old, E);
                 __syncthreads();  // the previous chunk is summed
                 mixed_slot_apply(E, beta, n_obs_e, n_mot_e, ys, tid);
@@ -578,11 +582,21 @@ int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, 
     return GSLAM_OK;
 }
 
+// The operator kernel of this graph: the nontemporal variant where its slots
+// (44 B each: factors, entry and both poses) exceed the Infinity Cache.
+using MixedSpmvKernel = decltype(&mixed_spmv_kernel<false>);
+
+static MixedSpmvKernel mixed_spmv_kernel_for(const DeviceGraph &g)
+{
+    return streams_past_cache(44 * g.mt_slots) ? mixed_spmv_kernel<true> : mixed_spmv_kernel<false>;
+}
+
 static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
                        gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
     const unsigned int gs = mixed_spmv_grid(g), gu = grid_for(n, 1);
+    const MixedSpmvKernel spmv_kern = mixed_spmv_kernel_for(g);
     const int32_t every = o.refresh_every > 0 ? o.refresh_every : 64;
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
     double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
@@ -606,7 +620,7 @@ static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *
                 is_true = 1;
             }
             float *pold = g.p32[k & 1], *pnew = g.p32[(k + 1) & 1];
-            hipLaunchKernelGGL(mixed_spmv_kernel, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
+            hipLaunchKernelGGL(spmv_kern, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
                                g.mt_ext, g.mt_ent, g.mt_own, g.mt_oth, g.frec, g.mt_tiles, g.xptr,
                                g.h.n_obs_e, g.h.n_mot_e, static_cast<float>(g.anchor), g.z32, pold,
                                pnew, g.q32, k, is_true, o.tol, PR[k & 1],
@@ -647,13 +661,14 @@ int time_mixed_spmv(DeviceGraph &g, int reps, double *us)
     GSLAM_TRY(g.mixed_factors());
     const int32_t n = g.h.n;
     const unsigned int gs = mixed_spmv_grid(g), gu = grid_for(n, 1);
+    const MixedSpmvKernel spmv_kern = mixed_spmv_kernel_for(g);
     double *PR[2] = {g.part + kPartOffPR0, g.part + kPartOffPR1};
     double *PQ = g.part + kPartOffPQT;  // stride kSpmvMaxGrid
     hipLaunchKernelGGL(mixed_setup_kernel, dim3(gu), dim3(kThreads), 0, g.stream, g.bdiag, g.bval,
                        g.xi, n, 1, g.dinv32, g.x, g.r32, g.z32, g.p32[0], g.p32[1], PR[0],
                        g.status);
     auto spmv = [&]() {
-        hipLaunchKernelGGL(mixed_spmv_kernel, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
+        hipLaunchKernelGGL(spmv_kern, dim3(gs), dim3(kThreads), 0, g.stream, g.mt_info,
                            g.mt_ext, g.mt_ent, g.mt_own, g.mt_oth, g.frec, g.mt_tiles, g.xptr,
                            g.h.n_obs_e, g.h.n_mot_e, static_cast<float>(g.anchor), g.z32, g.p32[0],
                            g.p32[1], g.q32, 0, 1, 1e-30, PR[0], static_cast<int32_t>(gu), PR[1],
