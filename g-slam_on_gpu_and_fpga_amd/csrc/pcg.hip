@@ -143,7 +143,7 @@ struct SpmvTileRegs {
     double yz[3], yp[3], zo, po, xo;
 };
 
-template <bool kNt>
+template <bool kNt, bool kNtVec>
 __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_info,
                                       const int32_t *__restrict__ brow_ptr,
                                       const int32_t *__restrict__ tcol,
@@ -177,7 +177,7 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     const int64_t o = 3 * static_cast<int64_t>(orow) + static_cast<int>(threadIdx.x) % 3;
     T.zo = z[o];
     T.po = p_old[o];
-    T.xo = x[o];
+    T.xo = stream_load<kNtVec>(x + o);
     const double z0 = z[j], z1 = z[j + 1], z2 = z[j + 2];
     const double q0 = p_old[j], q1 = p_old[j + 1], q2 = p_old[j + 2];
 #pragma unroll
@@ -187,6 +187,7 @@ __device__ inline void spmv_tile_load(int32_t t, const int4 *__restrict__ tile_i
     }
 }
 
+template <bool kNtVec>
 __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
                                        const int32_t *__restrict__ bcol,
                                        const double *__restrict__ bval,
@@ -235,8 +236,8 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
         const int64_t o = 3 * static_cast<int64_t>(T.r0 + static_cast<int32_t>(threadIdx.x) / 3) + a;
         const double pi = T.zo + beta * T.po;
         p_new[o] = pi;
-        q[o] = acc;
-        if (apply_x) x[o] = T.xo + alpha_prev * T.po;
+        stream_store<kNtVec>(acc, q + o);
+        if (apply_x) stream_store<kNtVec>(T.xo + alpha_prev * T.po, x + o);
         pq = pi * acc;
     }
     return pq;
@@ -245,8 +246,9 @@ __device__ inline double spmv_tile_run(const SpmvTileRegs &T, double beta,
 // (at most 72 VGPRs: 7 workgroups per CU, 1792 on the chip, keep the 1697
 // tiles of the N = 100k grid city in one round; at 92 VGPRs, 5 per CU, the
 // kernel measured 16 us in a solve.  A 64-VGPR cap spills.)  kNt: the tile
-// copy is loaded nontemporally (it exceeds the Infinity Cache).
-template <bool kNt>
+// copy is loaded nontemporally (it exceeds the Infinity Cache); kNtVec: x
+// and q as well (the solver's vectors exceed it).
+template <bool kNt, bool kNtVec>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
 pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
                 const double *__restrict__ bval, const int4 *__restrict__ tile_info,
@@ -288,7 +290,7 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     const double rr_a = pr_cur[ia], rr_b = pr_cur[ib];
     const double rz_a = pr_cur[kPartStride + ia], rz_b = pr_cur[kPartStride + ib];
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
-    if (t0 < n_tiles) spmv_tile_load<kNt>(t0, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
+    if (t0 < n_tiles) spmv_tile_load<kNt, kNtVec>(t0, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
     const bool in_a = tid < np_cur, in_b = tid + kThreads < np_cur;
     double red_v[2];
     red_v[0] = (0.0 + (in_a ? rr_a : 0.0)) + (in_b ? rr_b : 0.0);
@@ -343,9 +345,9 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
     for (int32_t t = t0; t < n_tiles; t += gridDim.x) {
         if (t != t0) {
             __syncthreads();  // the previous tile is summed before ys is rewritten
-            spmv_tile_load<kNt>(t, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
+            spmv_tile_load<kNt, kNtVec>(t, tile_info, brow_ptr, tcol, tval, z, p_old, x, T);
         }
-        pq[0] += spmv_tile_run(T, beta, bcol, bval, z, p_old, p_new, q, x, apply_x, alpha_prev, ys);
+        pq[0] += spmv_tile_run<kNtVec>(T, beta, bcol, bval, z, p_old, p_new, q, x, apply_x, alpha_prev, ys);
     }
     write_partials<1>(pq, pq_out, kSpmvMaxGrid, t0);  // slot t0: the sum of tiles t0, t0 + grid, ...
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -358,9 +360,10 @@ pcg_spmv_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict_
 // same partials (the same bits as summing them again here, without the
 // loads), x/r/z update, r.r / r.z partials.  One thread per pose: its 21
 // loads (x, r, p, q, D^-1) are issued before the p.q partials are loaded and
-// reduced; D^-1 is stored column-major per entry ([c][3i+a]).  kNt: q and
-// D^-1 are loaded nontemporally (D^-1 exceeds the Infinity Cache).
-template <bool kNt>
+// reduced; D^-1 is stored column-major per entry ([c][3i+a]).  kNt: D^-1
+// and q are loaded nontemporally (D^-1 exceeds the Infinity Cache); kNtVec:
+// q and r (the solver's vectors exceed it).
+template <bool kNt, bool kNtVec>
 __global__ void __launch_bounds__(kThreads)
 pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
                   double *__restrict__ r, double *__restrict__ z, int32_t n, int32_t k,
@@ -376,8 +379,8 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
         const int64_t o = 3 * static_cast<int64_t>(ii);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            rp[c] = r[o + c];
-            qp[c] = stream_load<kNt>(q + o + c);
+            rp[c] = stream_load<kNtVec>(r + o + c);
+            qp[c] = stream_load<kNt || kNtVec>(q + o + c);
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -413,7 +416,7 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             rr[c] = rp[c] - alpha * qp[c];
-            r[o + c] = rr[c];
+            stream_store<kNtVec>(rr[c], r + o + c);
         }
         double zz[3];
 #pragma unroll
@@ -538,21 +541,28 @@ diff_kernel(const double *__restrict__ part, int32_t np, int32_t n, double *__re
 }
 
 // The SpMV and update kernels of this graph: nontemporal variants where the
-// tile copy (76 B per slot) or q and D^-1 (96 B per pose) exceed the
-// Infinity Cache.
-using SpmvKernel = decltype(&pcg_spmv_kernel<false>);
-using UpdateKernel = decltype(&pcg_update_kernel<false>);
+// tile copy (76 B per slot) or D^-1 (72 B per pose) exceeds the Infinity
+// Cache, and where the six FP64 solver vectors (144 B per pose) do.
+using SpmvKernel = decltype(&pcg_spmv_kernel<false, false>);
+using UpdateKernel = decltype(&pcg_update_kernel<false, false>);
+
+static bool vectors_stream(const DeviceGraph &g)
+{
+    return streams_past_cache(144 * static_cast<int64_t>(g.h.n));
+}
 
 static SpmvKernel spmv_kernel_for(const DeviceGraph &g)
 {
-    return streams_past_cache(76 * int64_t{kTileBlocks} * g.n_tiles) ? pcg_spmv_kernel<true>
-                                                                       : pcg_spmv_kernel<false>;
+    const bool mat = streams_past_cache(76 * int64_t{kTileBlocks} * g.n_tiles);
+    if (vectors_stream(g)) return mat ? pcg_spmv_kernel<true, true> : pcg_spmv_kernel<false, true>;
+    return mat ? pcg_spmv_kernel<true, false> : pcg_spmv_kernel<false, false>;
 }
 
 static UpdateKernel update_kernel_for(const DeviceGraph &g)
 {
-    return streams_past_cache(96 * static_cast<int64_t>(g.h.n)) ? pcg_update_kernel<true>
-                                                                : pcg_update_kernel<false>;
+    const bool dinv = streams_past_cache(72 * static_cast<int64_t>(g.h.n));
+    if (vectors_stream(g)) return dinv ? pcg_update_kernel<true, true> : pcg_update_kernel<false, true>;
+    return dinv ? pcg_update_kernel<true, false> : pcg_update_kernel<false, false>;
 }
 
 // The padded tile copy of the current block values (before each FP64 solve).

@@ -35,7 +35,10 @@ constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 // evicting the vectors that can (FP64 SpMV at N = 1M: 109 -> 96.5 us; FP32
 // SpMV at N = 4M: 202 -> 184 us).  Smaller arrays use ordinary loads, which
 // keep them cached across iterations (N = 100k: 10.4 us against 11.1 us
-// nontemporal).
+// nontemporal).  Once the FP64 solver's six vectors no longer fit either, the
+// ones that only their own rows' threads read and write (x, q, r) are
+// streamed nontemporally too, which leaves the cache to z and p, whose
+// entries the SpMV gathers (FP64 iteration at N = 4M: 589 -> 561 us).
 constexpr int64_t kInfinityCacheBytes = int64_t{256} << 20;
 
 inline bool streams_past_cache(int64_t bytes) { return bytes > kInfinityCacheBytes; }
@@ -45,6 +48,13 @@ __device__ inline T stream_load(const T *p)
 {
     if constexpr (kNt) return __builtin_nontemporal_load(p);
     else return *p;
+}
+
+template <bool kNt, typename T>
+__device__ inline void stream_store(T v, T *p)
+{
+    if constexpr (kNt) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 inline unsigned int grid_for(int64_t rows, int lanes)
