@@ -38,7 +38,8 @@ Also reported:
 * grid10k: the FP64 PCG GN run on grid-city N=10,000 (tol 1e-10, the
   reference stop rule within 20 rounds), with the device time per phase;
 * large: FP64 and mixed PCG on grid-city N=1,200,000 (HBM-bound) to a true
-  relative residual of 1e-5;
+  relative residual of 1e-5, with the FP64 SpMV's in-solve HBM fraction
+  (`large.roofline_hbm`);
 * adjoint: converged-pose adjoint at grid-city N=10,000 (forward, backward,
   central finite differences on sampled edges, optional Adam steps).
 
@@ -462,6 +463,18 @@ def park_section(args, gslam, G, np):
                              fp64_correction_iters=st["iters_fp64"], **acc, **kw)
             log(f"park {name}: {ms:.1f} ms per GN step (solve {st['ms']:.1f} ms, "
                 f"{st['iters']} its), rel dx err {acc['rel_dx_err']:.2e}")
+        # the FP64 solve with two launches per iteration beside the default
+        # one-launch schedule of graphs this size
+        g.set_poses(gc["poses"])
+        g.assemble()
+        st = g.solve("pcg64", args.park_tol, 400000, pcg_schedule="two_launch")
+        st = g.solve("pcg64", args.park_tol, 400000, pcg_schedule="two_launch")
+        out["pcg64_two_launch"] = dict(solve_ms=st["ms"], pcg_iters=st["iters"],
+                                       us_per_iter=1000.0 * st["ms"] / max(st["iters"], 1),
+                                       **accuracy(np, g.dx(), dx_ref))
+        out["pcg64"]["us_per_iter"] = 1000.0 * out["pcg64"]["solve_ms"] / max(out["pcg64"]["pcg_iters"], 1)
+        log(f"park pcg64 two launches per iteration: solve {st['ms']:.1f} ms, {st['iters']} its "
+            f"(one launch: {out['pcg64']['solve_ms']:.1f} ms, {out['pcg64']['pcg_iters']} its)")
     out["mixed_speedup_vs_fp64"] = out["pcg64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
     return out
 
@@ -477,12 +490,27 @@ def large_section(args, gslam, G, np):
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
         g.assemble()
         for name in ("pcg64", "mixed"):
-            st = g.solve(name, args.large_tol, 200000)
+            kw = dict(profile_every=args.profile_every) if name == "pcg64" else {}
+            st = g.solve(name, args.large_tol, 200000, **kw)
             out[name] = dict(solve_ms=st["ms"], pcg_iters=st["iters"], converged=st["converged"],
                              true_rel_res=st["true_rel_res"],
                              us_per_iter=1000.0 * st["ms"] / max(st["iters"], 1))
             log(f"large N={args.large_n} {name}: {st['ms']:.0f} ms, {st['iters']} its, true rel res "
                 f"{st['true_rel_res']:.2e}")
+            if name == "pcg64" and st["spmv_us"] > 0:
+                # the dominant kernel where the matrix streams from HBM: HIP
+                # events around sampled SpMV launches of this solve
+                byts = spmv_bytes(g.info["n_poses"], g.info["n_blocks"])
+                gbs = byts / (st["spmv_us"] * 1e-6) / 1e9
+                out["roofline_hbm"] = dict(
+                    bound="hbm", kernel="pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)",
+                    regime=f"HBM-bound: {byts / 1e6:.0f} MB per launch, beyond the 256 MB "
+                           f"Infinity Cache", us_per_launch=st["spmv_us"],
+                    update_us_per_launch=st["update_us"], bytes_per_launch=byts,
+                    achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=gbs / HBM_PEAK_GBS,
+                    how=f"HIP events around every {args.profile_every}th SpMV launch of the solve")
+                log(f"large N={args.large_n} spmv64 in solve: {st['spmv_us']:.1f} us/launch, "
+                    f"{gbs:.0f} GB/s = {100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak")
     out["mixed_speedup_vs_fp64"] = out["pcg64"]["solve_ms"] / out["mixed"]["solve_ms"]
     return out
 
@@ -496,27 +524,34 @@ def grid10k_section(args, gslam, G, np):
     gc = G.grid_city(args.grid10k_n, seed=0)
     out = dict(n_poses=args.grid10k_n, lambda_=0.1, tol=1e-10)
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
-        g.gn_step("pcg64", 1e-10, 200000)          # warm-up round (allocations, code load)
-        g.set_poses(gc["poses"])
-        hip_sync()
-        t = time.perf_counter()
-        its, diffs, phases = [], [], []
-        for _ in range(20):
-            st, diff = g.gn_step("pcg64", 1e-10, 200000)
-            its.append(st["iters"])
-            diffs.append(diff)
-            phases.append((st["ms_assembly"], st["ms_setup"], st["ms"], st["ms_update"]))
-            if diff < 0.01:
-                break
-        hip_sync()
-        ms = 1000.0 * (time.perf_counter() - t)
-    ph = np.mean(np.array(phases), axis=0)
-    out.update(rounds=len(its), ms_run=ms, ms_per_step=ms / len(its), pcg_iters=int(sum(its)),
-               us_per_iter=1000.0 * float(np.sum(np.array(phases)[:, 2])) / max(sum(its), 1),
-               last_diff=diffs[-1], phase_ms_mean=dict(assembly=ph[0], setup=ph[1], solve=ph[2],
-                                                        update=ph[3]))
-    log(f"grid10k pcg64: {len(its)} GN rounds in {ms:.1f} ms ({ms / len(its):.1f} ms per step, "
-        f"{sum(its)} PCG its, last diff {diffs[-1]:.2e})")
+        for schedule in ("two_launch", "auto"):     # the default ('auto') last
+            g.set_poses(gc["poses"])
+            g.gn_step("pcg64", 1e-10, 200000, pcg_schedule=schedule)   # warm-up round
+            g.set_poses(gc["poses"])
+            hip_sync()
+            t = time.perf_counter()
+            its, diffs, phases = [], [], []
+            for _ in range(20):
+                st, diff = g.gn_step("pcg64", 1e-10, 200000, pcg_schedule=schedule)
+                its.append(st["iters"])
+                diffs.append(diff)
+                phases.append((st["ms_assembly"], st["ms_setup"], st["ms"], st["ms_update"]))
+                if diff < 0.01:
+                    break
+            hip_sync()
+            ms = 1000.0 * (time.perf_counter() - t)
+            ph = np.mean(np.array(phases), axis=0)
+            res = dict(rounds=len(its), ms_run=ms, ms_per_step=ms / len(its), pcg_iters=int(sum(its)),
+                       us_per_iter=1000.0 * float(np.sum(np.array(phases)[:, 2])) / max(sum(its), 1),
+                       last_diff=diffs[-1], phase_ms_mean=dict(assembly=ph[0], setup=ph[1],
+                                                               solve=ph[2], update=ph[3]))
+            log(f"grid10k pcg64 ({schedule}): {len(its)} GN rounds in {ms:.1f} ms "
+                f"({ms / len(its):.1f} ms per step, {sum(its)} PCG its, "
+                f"{res['us_per_iter']:.2f} us per iteration, last diff {diffs[-1]:.2e})")
+            if schedule == "auto":
+                out.update(res)
+            else:
+                out["two_launch"] = res
     return out
 
 
@@ -639,6 +674,18 @@ def profile_kernel(args, gslam, G):
                 f"{byts / (us * 1e-6) / 1e9:.0f} GB/s = "
                 f"{100 * byts / (us * 1e-6) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak")
         return
+    if args.profile_kernel == "pcg_solve":
+        # whole FP64 PCG solves (tol 1e-10) with the chosen launch schedule
+        gc = G.grid_city(args.n, seed=args.seed) if args.graph == "grid" else \
+            G.park(args.n, seed=0, route="laps")
+        lam = 0.1 if args.graph == "grid" else 1.0
+        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, lam)) as g:
+            g.assemble()
+            for _ in range(3):
+                st = g.solve("pcg64", 1e-10, 400000, pcg_schedule=args.pcg_schedule)
+                log(f"pcg_solve {args.graph} N={args.n} ({args.pcg_schedule}): {st['ms']:.2f} ms, "
+                    f"{st['iters']} its, {1000 * st['ms'] / max(st['iters'], 1):.2f} us per iteration")
+        return
     if args.profile_kernel in ("spmv64", "update64"):
         k = args.profile_kernel
         gc = G.grid_city(args.n, seed=args.seed)
@@ -703,7 +750,11 @@ def main():
     ap.add_argument("--adj-n", type=int, default=10000, help="adjoint section size (0 skips)")
     ap.add_argument("--adj-fd", type=int, default=8, help="edges checked by central differences")
     ap.add_argument("--adam", type=int, default=0, help="Adam steps on (w_e, log lambda)")
-    ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "spmv_mixed", "gj_trailing",
+    ap.add_argument("--graph", choices=["grid", "park"], default="grid",
+                    help="--profile-kernel pcg_solve: grid city or the park graph (laps route)")
+    ap.add_argument("--pcg-schedule", choices=["auto", "two_launch"], default="auto",
+                    help="--profile-kernel pcg_solve: FP64 PCG launch schedule")
+    ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "spmv_mixed", "pcg_solve", "gj_trailing",
                                                  "assembly", "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "
                          "passes; 'copy' is a 64 MiB device copy for calibrating the counters, "

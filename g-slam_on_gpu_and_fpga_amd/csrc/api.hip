@@ -72,6 +72,18 @@ int DeviceGraph::tval_reserve()
     return alloc(&tval, static_cast<int64_t>(9 * kTileBlocks) * n_tiles);
 }
 
+int DeviceGraph::fused_reserve()
+{
+    if (fpart) return GSLAM_OK;
+    GSLAM_TRY(alloc(&r_alt, dim()));
+    GSLAM_TRY(alloc(&q_alt, dim()));
+    GSLAM_TRY(alloc(&zwp, 2 * 3 * dim()));
+    double *f = nullptr;
+    GSLAM_TRY(alloc(&f, static_cast<int64_t>(2 * kFusedParts) * kFusedMaxTiles));
+    fpart = f;  // last: fpart != nullptr means every fused buffer exists
+    return GSLAM_OK;
+}
+
 int DeviceGraph::mixed_factors()
 {
     if (factors_valid) return GSLAM_OK;
@@ -263,6 +275,7 @@ void gslam_default_opts(gslam_solve_opts *o)
     o->refine_mode = GSLAM_REFINE_IR;
     o->profile_every = 0;
     o->mixed_handover = 0.0;
+    o->pcg_schedule = GSLAM_PCG_AUTO;
 }
 
 void gslam_default_desc(gslam_graph_desc *d)
@@ -486,6 +499,10 @@ static int check_opts(const gslam_solve_opts *o)
     }
     if (!(o->mixed_handover >= 0.0)) {
         set_error("mixed_handover must be >= 0");
+        return GSLAM_ERR_ARG;
+    }
+    if (o->pcg_schedule != GSLAM_PCG_AUTO && o->pcg_schedule != GSLAM_PCG_TWO_LAUNCH) {
+        set_error("pcg_schedule must be GSLAM_PCG_AUTO or GSLAM_PCG_TWO_LAUNCH");
         return GSLAM_ERR_ARG;
     }
     if (o->solver == GSLAM_SOLVER_GJ_MIXED &&

@@ -153,6 +153,12 @@ struct DeviceGraph {
     int vecs_reserve();
     int csr_tmp_reserve();
     int tval_reserve();
+    // One-launch-per-iteration FP64 PCG of small graphs (pcg.hip): second
+    // buffers of r and q (3N each), per pose and parity the record {z, w =
+    // D^-1 q, p} (9 doubles) the SpMV gathers, and the partials of two
+    // launches; allocated by its first solve.
+    double *r_alt = nullptr, *q_alt = nullptr, *zwp = nullptr, *fpart = nullptr;
+    int fused_reserve();
     // Makes frec valid for the assembled system: allocates the mixed solver's
     // buffers on first use and, if the last assembly ran without them,
     // re-runs it (the assembly is deterministic: bval and xi keep their bits).

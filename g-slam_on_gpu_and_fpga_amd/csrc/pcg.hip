@@ -27,6 +27,7 @@
 #include "pcg_common.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 
 namespace gslam {
@@ -430,6 +431,281 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
     write_partials<2>(v, pr_out, kPartStride);
 }
 
+// One launch L per PCG iteration for graphs of at most kFusedMaxTiles SpMV
+// tiles (one workgroup per tile), where two launches per iteration, not the
+// bytes, bound the solve.  Launch L completes iteration L-1 and starts
+// iteration L of the recurrence above:
+//   from the partials of launch L-1 -- p.q, r.z, q.z, q.w and r.r of
+//   p_(L-1), q_(L-1) = A p_(L-1), r_(L-1), z_(L-1) = D^-1 r_(L-1) and
+//   w_(L-1) = D^-1 q_(L-1) --, reduced by every workgroup in the same order:
+//     stop if ||r_(L-1)|| < tol (L >= 2: the test after each update,
+//     MatrixOps.c:366), or if p.q == 0 (MatrixOps.c:357);
+//     alpha = r.z / p.q;
+//     r.z_L = r.z - 2 alpha q.z + alpha^2 q.w, which is r_L . D^-1 r_L for
+//       r_L = r_(L-1) - alpha q_(L-1) (D^-1 is symmetric); beta = r.z_L / r.z;
+//   then p_L = (z_(L-1) - alpha w_(L-1)) + beta p_(L-1) for every gathered
+//   column (its record {z, w, p} of launch L-1: one 72-byte run per pose) and
+//   its own rows, the same arithmetic everywhere; for its own rows r_L,
+//   x_L = x_(L-1) + alpha p_(L-1), q_L = A p_L, z_L = D^-1 r_L and
+//   w_L = D^-1 q_L (the pose's three entries through LDS), the record of
+//   launch L and its partials.
+// z_(L-1) - alpha w_(L-1) is D^-1 r_L up to the rounding of this one step
+// (z is recomputed from r by every launch, so nothing accumulates); r.z_L is
+// reduced from the recomputed z_L.  Launch 0 starts from r_(-1) = b,
+// q_(-1) = 0, the record {D^-1 b, 0, 0} and x = 0 (pcg_fused_setup_kernel)
+// with alpha = beta = 0.  r, q and the records are double-buffered by the
+// parity of L (other workgroups gather launch L-1's while this one writes
+// launch L's).
+//
+// beta comes from the one-step identity instead of a reduction of r_L . z_L,
+// which would need a second grid-wide synchronisation; anchored on the reduced
+// r.z_(L-1) every iteration, its rounding does not accumulate.  CPU study (the
+// same block-Jacobi PCG in FP64, iterations to the tolerance and rel dx error
+// against the refined direct solve, two-launch recurrence in brackets):
+// grid-city N=10k at tol 1e-13 11,889 its, 6.5e-10 (11,967, 7.6e-10); park
+// N=7120 1,500, 3.7e-13 (1,499, 2.6e-13); grid-city N=100k at 3e-13 153,799,
+// 7.3e-9 (153,778, 5.7e-9).
+__global__ void __launch_bounds__(kThreads)
+pcg_fused_setup_kernel(const int32_t *__restrict__ bdiag, const double *__restrict__ bval,
+                       const double *__restrict__ b, int32_t n, int32_t max_iter,
+                       double *__restrict__ dinv, double *__restrict__ x,
+                       double *__restrict__ r, double *__restrict__ q,
+                       double *__restrict__ rec, double *__restrict__ part_out,
+                       PcgStatus *status)
+{
+    double v[1] = {0.0};
+    const int64_t dim = 3 * static_cast<int64_t>(n);
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a[9], inv[9];
+        const double *blk = bval + 9 * static_cast<int64_t>(bdiag[i]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k] = blk[k];
+        inv3_cofactor(a, inv);
+        const int64_t o = 3 * static_cast<int64_t>(i);
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dinv[c * dim + o + aa] = inv[3 * aa + c];
+        const double b0 = b[o], b1 = b[o + 1], b2 = b[o + 2];
+        double *e = rec + 9 * static_cast<int64_t>(i);
+        e[0] = fma(inv[2], b2, fma(inv[1], b1, inv[0] * b0));
+        e[1] = fma(inv[5], b2, fma(inv[4], b1, inv[3] * b0));
+        e[2] = fma(inv[8], b2, fma(inv[7], b1, inv[6] * b0));
+#pragma unroll
+        for (int k = 3; k < 9; ++k) e[k] = 0.0;
+        x[o] = 0.0; x[o + 1] = 0.0; x[o + 2] = 0.0;
+        r[o] = b0; r[o + 1] = b1; r[o + 2] = b2;
+        q[o] = 0.0; q[o + 1] = 0.0; q[o + 2] = 0.0;
+        v[0] += b0 * b0 + b1 * b1 + b2 * b2;   // ||r0||^2 = ||b||^2
+    }
+    write_partials<1>(v, part_out, kPartStride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        status->done = max_iter <= 0 ? 1 : 0;
+        status->converged = 0;
+        status->iters = 0;
+        status->flags = 0;
+        status->refreshes = 0;
+        status->x_iter = -1;
+        status->rnorm = 0.0;
+        status->bnorm = 0.0;
+        status->tol = 0.0;
+    }
+}
+
+// p_L of one entry from its record {z, w, p} of launch L-1 (explicit fma, so
+// that every workgroup that forms the entry rounds alike).
+__device__ inline double fused_p(double z, double w, double p_old, double alpha, double beta)
+{
+    return fma(beta, p_old, fma(-alpha, w, z));
+}
+
+__global__ void __launch_bounds__(kThreads)
+pcg_fused_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
+                 const double *__restrict__ bval, const int4 *__restrict__ tile_info,
+                 const int32_t *__restrict__ tcol, const double *__restrict__ tval,
+                 const double *__restrict__ dinv, int32_t n, const double *__restrict__ r_in,
+                 const double *__restrict__ q_in, const double *__restrict__ rec_in,
+                 double *__restrict__ r_out, double *__restrict__ q_out,
+                 double *__restrict__ rec_out, double *__restrict__ x, int32_t L, double tol_rel,
+                 const double *__restrict__ part_in, int32_t np_in, double *__restrict__ part_out,
+                 PcgStatus *status)
+{
+    static_assert(kTileBlocks == kThreads, "one lane per padded block slot");
+    static_assert(kFusedMaxTiles <= 2 * kThreads, "two partials per thread and quantity");
+    __shared__ double ys[3 * kTileBlocks];
+    __shared__ double xs[2][kThreads];  // the pose's r_L, then q_L entries
+    __shared__ double red[kFusedParts][kThreads / kWave];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int32_t t = static_cast<int32_t>(blockIdx.x);
+    const int64_t dim = 3 * static_cast<int64_t>(n);
+    const int32_t done_in = status->done;
+    const double tol_in = status->tol;
+    // the partials of launch L-1 first (launch 0: ||b||^2 of the setup
+    // kernel, np_in <= 2 x kThreads), as unconditional loads from clamped
+    // indices, summed below in the same order in every workgroup
+    const int ia = min(tid, np_in - 1), ib = min(tid + kThreads, np_in - 1);
+    const bool in_a = tid < np_in, in_b = tid + kThreads < np_in;
+    double pa[kFusedParts], pb[kFusedParts];
+#pragma unroll
+    for (int k = 0; k < kFusedParts; ++k) {
+        const int64_t off = L == 0 ? 0 : static_cast<int64_t>(k) * kFusedMaxTiles;
+        pa[k] = part_in[off + ia];
+        pb[k] = part_in[off + ib];
+    }
+    asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
+    // this lane's block (tile-padded slot: zeros and a valid column in padding)
+    // and the gathered record of its column
+    const int4 ti = tile_info[t];
+    const int64_t slot = static_cast<int64_t>(t) * kTileBlocks + tid;
+    const double *ge = rec_in + 9 * static_cast<int64_t>(tcol[slot]);
+    const double *tv = tval + 9 * static_cast<int64_t>(t) * kTileBlocks + tid;
+    double v[9], gz[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = tv[k * kTileBlocks];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gz[k] = ge[k];
+    // this thread's output entry (row a of pose orow, clamped into the tile)
+    const int a = tid % 3;
+    const bool has_blk = ti.z + tid < ti.w, has_out = tid < 3 * (ti.y - ti.x);
+    const int32_t orow = min(ti.x + tid / 3, ti.y - 1);
+    const int32_t ob0 = brow_ptr[orow], ob1 = brow_ptr[orow + 1];
+    const int64_t o = 3 * static_cast<int64_t>(orow) + a;
+    const double *oe = rec_in + 9 * static_cast<int64_t>(orow);
+    const double oz = oe[a], ow = oe[3 + a], op = oe[6 + a];
+    const double orr = r_in[o], oq = q_in[o], ox = x[o];
+    const double od0 = dinv[o], od1 = dinv[dim + o], od2 = dinv[2 * dim + o];  // row a of D^-1
+    double pv[kFusedParts];
+#pragma unroll
+    for (int k = 0; k < kFusedParts; ++k) pv[k] = (0.0 + (in_a ? pa[k] : 0.0)) + (in_b ? pb[k] : 0.0);
+    block_allsum<kFusedParts, kThreads>(pv, red);
+    if (done_in) return;
+    const double pq = pv[0], rz = pv[1], qz = pv[2], qw = pv[3], rr = L == 0 ? pv[0] : pv[4];
+    double alpha = 0.0, beta = 0.0;
+    const bool lead = t == 0 && tid == 0;
+    if (L == 0) {
+        if (lead) {
+            status->bnorm = sqrt(rr);
+            status->tol = tol_rel * sqrt(rr);
+            status->rnorm = sqrt(rr);
+        }
+    } else {
+        if (!isfinite(rr) || !isfinite(rz)) {  // NaN / Inf in the residual (flag bit 1)
+            if (lead) {
+                status->done = 1;
+                status->flags |= 2;
+                status->iters = L - 1;
+            }
+            return;
+        }
+        if (L >= 2 && sqrt(rr) < tol_in) {  // the test after update L-1
+            if (lead) {
+                status->done = 1;
+                status->converged = 1;
+                status->iters = L - 1;
+                status->rnorm = sqrt(rr);
+            }
+            return;
+        }
+        if (pq == 0.0 || !isfinite(pq)) {  // MatrixOps.c:357 (flag bit 0), or NaN / Inf
+            if (lead) {
+                status->done = 1;
+                status->flags |= pq == 0.0 ? 1 : 2;
+                status->iters = L - 1;
+            }
+            return;
+        }
+        alpha = rz / pq;
+        beta = fma(alpha * alpha, qw, fma(-2.0 * alpha, qz, rz)) / rz;
+    }
+    // p_L of the gathered column, its products into LDS
+    if (has_blk) {
+        double pj[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pj[c] = fused_p(gz[c], gz[3 + c], gz[6 + c], alpha, beta);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            ys[r * kTileBlocks + tid] = v[3 * r] * pj[0] + v[3 * r + 1] * pj[1] + v[3 * r + 2] * pj[2];
+    }
+    // own entry: r_L (to LDS for z_L = D^-1 r_L), p_L
+    const double r_a = fma(-alpha, oq, orr);
+    const double p_a = fused_p(oz, ow, op, alpha, beta);
+    xs[0][tid] = r_a;
+    __syncthreads();
+    double acc = 0.0;
+    if (has_out) {
+        const int32_t hi = min(ob1, ti.z + kTileBlocks);
+        for (int32_t c = max(ob0, ti.z); c < hi; ++c) acc += ys[a * kTileBlocks + (c - ti.z)];
+    }
+    // a single block row longer than one tile: further chunks from the BSR
+    // arrays (the condition is the same in the whole workgroup)
+    if (ti.w > ti.z + kTileBlocks) __syncthreads();
+    for (int32_t cb = ti.z + kTileBlocks; cb < ti.w; cb += kTileBlocks) {
+        const int32_t bb = cb + tid;
+        if (bb < ti.w) {
+            const double *ee = rec_in + 9 * static_cast<int64_t>(bcol[bb]);
+            double pj[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pj[c] = fused_p(ee[c], ee[3 + c], ee[6 + c], alpha, beta);
+            const double *bv = bval + 9 * static_cast<int64_t>(bb);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                ys[r * kTileBlocks + tid] = bv[3 * r] * pj[0] + bv[3 * r + 1] * pj[1] + bv[3 * r + 2] * pj[2];
+        }
+        __syncthreads();
+        if (has_out) {
+            const int32_t lo = max(ob0, cb), hi = min(ob1, cb + kTileBlocks);
+            for (int32_t c = lo; c < hi; ++c) acc += ys[a * kTileBlocks + (c - cb)];
+        }
+        __syncthreads();
+    }
+    // q_L to LDS for w_L = D^-1 q_L; both D^-1 products from the pose's three
+    // entries (tid - a is the pose's first thread)
+    xs[1][tid] = acc;
+    __syncthreads();
+    double part[kFusedParts] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (has_out) {
+        const int base = tid - a;
+        const double z_a = fma(od2, xs[0][base + 2], fma(od1, xs[0][base + 1], od0 * xs[0][base]));
+        const double w_a = fma(od2, xs[1][base + 2], fma(od1, xs[1][base + 1], od0 * xs[1][base]));
+        r_out[o] = r_a;
+        q_out[o] = acc;
+        double *re = rec_out + 9 * static_cast<int64_t>(orow);
+        re[a] = z_a;
+        re[3 + a] = w_a;
+        re[6 + a] = p_a;
+        if (L > 0) x[o] = fma(alpha, op, ox);
+        part[0] = p_a * acc;
+        part[1] = r_a * z_a;
+        part[2] = acc * z_a;
+        part[3] = acc * w_a;
+        part[4] = r_a * r_a;
+    }
+    write_partials<kFusedParts>(part, part_out, kFusedMaxTiles, t);
+    if (lead) status->iters = L;
+}
+
+// After the last launch of a chunk (launches = L + 1): the stopping test on
+// r_L for the host, and the iteration cap.
+__global__ void __launch_bounds__(kThreads)
+pcg_fused_finalize_kernel(const double *__restrict__ part_last, int32_t np, int32_t launches,
+                          int32_t max_iter, PcgStatus *status)
+{
+    if (status->done || launches < 2) return;  // launch 0's partials hold r_0 = b
+    double cur[1];
+    sum_partials<1>(part_last + 4 * kFusedMaxTiles, np, kFusedMaxTiles, cur);
+    if (threadIdx.x == 0) {
+        const int32_t it = launches - 1;
+        status->iters = it;
+        status->rnorm = sqrt(cur[0]);
+        if (sqrt(cur[0]) < status->tol) {
+            status->done = 1;
+            status->converged = 1;
+        } else if (it >= max_iter) {
+            status->done = 1;
+        }
+    }
+}
+
 // After the last launched iteration (or at a poll): the stopping test on the
 // newest partials, for the host.
 __global__ void __launch_bounds__(kThreads)
@@ -621,9 +897,96 @@ int pose_update(DeviceGraph &g, double *diff_out)
     return GSLAM_OK;
 }
 
+// One kernel per iteration (pcg_fused_kernel) for graphs of at most
+// kFusedMaxTiles tiles.
+static int pcg_solve_fused(DeviceGraph &g, const gslam_solve_opts &o, const double *b,
+                           double *x_out, gslam_solve_stats &s)
+{
+    const int32_t n = g.h.n;
+    const unsigned int ge = grid_for(n, 1);
+    const unsigned int gt = static_cast<unsigned int>(g.n_tiles);
+    PcgStatus *st = g.status;
+    GSLAM_TRY(g.tval_reserve());  // first solve: outside the timed region
+    GSLAM_TRY(g.fused_reserve());
+    double *R[2] = {g.r, g.r_alt}, *Q[2] = {g.q, g.q_alt};
+    double *E[2] = {g.zwp, g.zwp + 3 * g.dim()};
+    double *FP[2] = {g.fpart, g.fpart + kFusedParts * kFusedMaxTiles};
+    GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
+    GSLAM_TRY(launch_tile_pack(g));
+    // launch 0 reads r_(-1) = b, q_(-1) = 0, the record {D^-1 b, 0, 0} (parity
+    // 1) and x = 0, and ||b||^2 from the partials of the setup kernel
+    hipLaunchKernelGGL(pcg_fused_setup_kernel, dim3(ge), dim3(kThreads), 0, g.stream, g.bdiag,
+                       g.bval, b, n, o.max_iter, g.dinv, x_out, R[1], Q[1], E[1],
+                       g.part + kPartOffPR0, st);
+    GSLAM_HIP(hipGetLastError());
+    GSLAM_HIP(hipEventRecord(g.ev2, g.stream));  // end of the setup phase
+    constexpr int kMaxSamples = 128;
+    int n_samp = 0;
+    if (o.profile_every > 0 && g.prof_ev.empty()) {
+        g.prof_ev.resize(4 * kMaxSamples, nullptr);
+        for (hipEvent_t &e : g.prof_ev) GSLAM_HIP(hipEventCreate(&e));
+    }
+    int32_t k = 0;  // launches enqueued: launch L performs update L (L >= 1)
+    auto enqueue = [&](int32_t m) {
+        for (int32_t c = 0; c < m; ++c, ++k) {
+            const int cur = k & 1, prev = cur ^ 1;
+            const double *pin = k == 0 ? g.part + kPartOffPR0 : FP[prev];
+            const int32_t npin = k == 0 ? static_cast<int32_t>(ge) : static_cast<int32_t>(gt);
+            const bool samp = o.profile_every > 0 && k % o.profile_every == 0 && n_samp < kMaxSamples;
+            auto launch = [&](auto... a) {
+                if (samp) {
+                    hipEvent_t *ev = &g.prof_ev[4 * n_samp++];
+                    hipExtLaunchKernelGGL(pcg_fused_kernel, dim3(gt), dim3(kThreads), 0, g.stream, ev[0],
+                                          ev[1], 0, a...);
+                } else {
+                    hipLaunchKernelGGL(pcg_fused_kernel, dim3(gt), dim3(kThreads), 0, g.stream, a...);
+                }
+            };
+            launch(g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.dinv, n, R[prev], Q[prev],
+                   E[prev], R[cur], Q[cur], E[cur], x_out, k, o.tol, pin, npin, FP[cur], st);
+        }
+        hipLaunchKernelGGL(pcg_fused_finalize_kernel, dim3(1), dim3(kThreads), 0, g.stream,
+                           FP[(k - 1) & 1], static_cast<int32_t>(gt), k, o.max_iter, st);
+    };
+    // max_iter updates take max_iter + 1 launches
+    gslam_solve_opts oc = o;
+    oc.max_iter = o.max_iter < INT32_MAX ? o.max_iter + 1 : o.max_iter;
+    GSLAM_TRY(run_chunked(g, oc, enqueue, k));
+    GSLAM_HIP(hipEventRecord(g.ev1, g.stream));
+    GSLAM_HIP(hipMemcpyAsync(g.status_host, st, sizeof(PcgStatus), hipMemcpyDeviceToHost, g.stream));
+    GSLAM_HIP(hipStreamSynchronize(g.stream));
+    float ms = 0.f, ms_setup = 0.f;
+    GSLAM_HIP(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+    GSLAM_HIP(hipEventElapsedTime(&ms_setup, g.ev0, g.ev2));
+    s.ms_setup = ms_setup;
+    const PcgStatus &ps = *g.status_host;
+    s.iters = ps.iters;
+    s.converged = ps.converged;
+    s.flags = ps.flags;
+    s.reliable_updates = 0;
+    s.rel_res = ps.bnorm > 0 ? ps.rnorm / ps.bnorm : ps.rnorm;
+    s.ms = ms;
+    s.spmv_us = s.update_us = 0.0;
+    if (n_samp > 0) {
+        // samples past the converged iteration timed early-exit launches
+        const int use = std::max(1, std::min(n_samp, (ps.iters + o.profile_every - 1) / o.profile_every));
+        double a = 0.0;
+        for (int i = 0; i < use; ++i) {
+            float t0 = 0.f;
+            GSLAM_HIP(hipEventElapsedTime(&t0, g.prof_ev[4 * i], g.prof_ev[4 * i + 1]));
+            a += t0;
+        }
+        s.spmv_us = 1000.0 * a / use;  // the whole iteration's one launch
+    }
+    GSLAM_TRY(true_rel_residual(g, b, x_out, &s.true_rel_res));
+    return GSLAM_OK;
+}
+
 int pcg_solve_fp64(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
                    gslam_solve_stats &s)
 {
+    if (o.pcg_schedule == GSLAM_PCG_AUTO && g.n_tiles <= kFusedMaxTiles)
+        return pcg_solve_fused(g, o, b, x_out, s);
     const int32_t n = g.h.n;
     const unsigned int ge = grid_for(n, 1);
     const unsigned int gt = static_cast<unsigned int>(std::min<int32_t>(g.n_tiles, kSpmvMaxGrid));

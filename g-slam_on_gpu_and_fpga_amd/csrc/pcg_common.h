@@ -28,6 +28,13 @@ constexpr int kPartTotal = kPartOffRz + 8;
 
 constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 
+// The FP64 PCG of graphs with at most kFusedMaxTiles SpMV tiles (N up to
+// ~30k) runs one kernel per iteration: every workgroup re-reads all
+// kFusedParts partials of the previous launch (at most 10 loads per
+// thread), so the update of r, z, p and x needs no launch of its own.
+constexpr int kFusedMaxTiles = 512;
+constexpr int kFusedParts = 5;  // p.q, r.z, q.z, q.D^-1 q, r.r
+
 // Arrays that a PCG kernel reads once per iteration (the matrix values and
 // column indices, the FP32 factors, D^-1) are loaded nontemporally when they
 // are larger than the 256 MB MI355X Infinity Cache: they cannot stay there

@@ -59,7 +59,8 @@ class _Opts(C.Structure):
     _fields_ = [("solver", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32),
                 ("refine_sweeps", C.c_int32), ("refresh_every", C.c_int32),
                 ("check_every", C.c_int32), ("refine_mode", C.c_int32),
-                ("profile_every", C.c_int32), ("mixed_handover", C.c_double)]
+                ("profile_every", C.c_int32), ("mixed_handover", C.c_double),
+                ("pcg_schedule", C.c_int32)]
 
 
 class _Stats(C.Structure):
@@ -239,8 +240,12 @@ def resume_log(log_path: str, dump_path: str, out_path: str) -> int:
     return n
 
 
+PCG_SCHEDULES = {"auto": 0, "two_launch": 1}
+
+
 def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_every=64,
-              check_every=0, refine_mode="ir", profile_every=0, mixed_handover=0.0) -> _Opts:
+              check_every=0, refine_mode="ir", profile_every=0, mixed_handover=0.0,
+              pcg_schedule="auto") -> _Opts:
     o = _Opts()
     lib().gslam_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -252,6 +257,8 @@ def make_opts(solver="pcg64", tol=1e-6, max_iter=500, refine_sweeps=3, refresh_e
     o.refine_mode = REFINE[refine_mode] if isinstance(refine_mode, str) else int(refine_mode)
     o.profile_every = profile_every
     o.mixed_handover = mixed_handover
+    o.pcg_schedule = (PCG_SCHEDULES[pcg_schedule] if isinstance(pcg_schedule, str)
+                      else int(pcg_schedule))
     return o
 
 

@@ -125,7 +125,20 @@ typedef struct {
                                    Omega d = xi - Omega dx to tol (FP64
                                    iterative refinement); 0 = FP32 iterations
                                    with FP64 refreshes down to tol           */
+    int32_t pcg_schedule;       /* PCG_FP64 (and the mixed solver's FP64
+                                   correction): GSLAM_PCG_AUTO or
+                                   GSLAM_PCG_TWO_LAUNCH                      */
 } gslam_solve_opts;
+
+/* FP64 PCG launch schedules (same block-Jacobi PCG, same stopping rule):
+ *   AUTO: one kernel per iteration on graphs of at most 512 SpMV tiles (about
+ *         30,000 poses), where kernel launches bound the iteration; beta then
+ *         comes from the one-step identity r.z' = r.z - 2 alpha q.z +
+ *         alpha^2 q.D^-1 q, so the iterates differ from the two-launch ones
+ *         in the last bits; two kernels per iteration on larger graphs.
+ *   TWO_LAUNCH: two kernels per iteration at every size. */
+#define GSLAM_PCG_AUTO       0
+#define GSLAM_PCG_TWO_LAUNCH 1
 
 /* GJ_MIXED refinement on the true residual xi - Omega dx (FP64), with the
  * FP32 blocked inverse M of the Jacobi-scaled Omega as the approximate

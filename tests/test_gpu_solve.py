@@ -53,26 +53,34 @@ def test_pcg64_reference_config_on_reference_system(oracle, key, bound):
     assert st["true_rel_res"] <= 1.01 * max(st["rel_res"], 1e-6) or not st["converged"]
 
 
+SCHEDULES = ["auto", "two_launch"]
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("name", ["grid800", "grid5000", "park2000"])
-def test_pcg64_converged_against_direct(name):
+def test_pcg64_converged_against_direct(name, schedule):
+    """Both launch schedules (on these graphs 'auto' is one launch per
+    iteration) meet 1e-8 against the direct solve."""
     gc = {"grid800": lambda: G.grid_city(800, seed=1), "grid5000": lambda: G.grid_city(5000, seed=2),
           "park2000": lambda: G.park(2000, seed=3)}[name]()
     lam = 1.0 if name.startswith("park") else 0.1
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, lam)) as g:
         g.assemble()
         xs = direct(*g.system_csr())
-        st = g.solve("pcg64", 1e-12, 200000)
+        st = g.solve("pcg64", 1e-12, 200000, pcg_schedule=schedule)
         # the stop test is on the recursively updated residual; the true
         # residual drifts from it at the rounding level (1.006e-12 seen)
         assert st["converged"] and st["true_rel_res"] < 2e-12
         assert rel(g.dx(), xs) <= 1e-8
 
 
-def test_pcg64_block_row_longer_than_a_tile():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_pcg64_block_row_longer_than_a_tile(schedule):
     """Pose 0 tied to every other pose by loop closures: its block row holds
-    700 blocks, more than one 256-block SpMV tile, so the FP64 SpMV walks it
-    in three chunks (the first from the padded tile slots, the rest from the
-    BSR arrays).  Converged FP64 PCG against the direct solve."""
+    700 blocks, more than one 256-block SpMV tile, so the FP64 SpMV (and the
+    one-launch iteration) walks it in three chunks (the first from the padded
+    tile slots, the rest from the BSR arrays).  Converged FP64 PCG against
+    the direct solve."""
     rng = np.random.default_rng(5)
     n = 700
     truth = np.zeros((n, 3))
@@ -84,7 +92,7 @@ def test_pcg64_block_row_longer_than_a_tile():
         rp, ci, va, xi = g.system_csr()
         assert rp[1] - rp[0] == 3 * n       # scalar row 0 spans all 700 block columns
         xs = direct(rp, ci, va, xi)
-        st = g.solve("pcg64", 1e-12, 200000)
+        st = g.solve("pcg64", 1e-12, 200000, pcg_schedule=schedule)
         assert st["converged"] and st["true_rel_res"] < 2e-12
         assert rel(g.dx(), xs) <= 1e-8
 
@@ -104,6 +112,41 @@ def test_pcg64_several_tiles_per_workgroup(oracle):
         x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
         assert it_o == 50
         assert rel(g.dx(), x_o) <= 1e-9
+
+
+def test_pcg64_one_launch_schedule_against_oracle(oracle):
+    """Grid city N = 20,000 (about 340 SpMV tiles: one launch per iteration
+    under 'auto').  Fifty iterations against fifty of the CPU oracle (the
+    reference's two-reduction recurrence) on the same system: the one-step
+    beta identity and the different summation orders stay at the rounding
+    level.  Then both schedules to tol 1e-12: iteration counts within 2% and
+    both within 1e-8 of the direct solve."""
+    gc = G.grid_city(20000, seed=8)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-30, 50, pcg_schedule=schedule)
+            assert st["iters"] == 50 and it_o == 50 and not st["converged"]
+            assert rel(g.dx(), x_o) <= 1e-9, schedule
+        xs = direct(rp, ci, va, xi)
+        its = {}
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-12, 400000, pcg_schedule=schedule)
+            assert st["converged"] and st["true_rel_res"] < 2e-12, schedule
+            assert rel(g.dx(), xs) <= 1e-8, schedule
+            its[schedule] = st["iters"]
+        assert abs(its["auto"] - its["two_launch"]) <= 0.02 * its["two_launch"], its
+
+
+def test_pcg_schedule_is_validated():
+    gc = G.grid_city(300, seed=4)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        with pytest.raises(gslam.GSlamError) as ei:
+            g.solve("pcg64", 1e-10, 1000, pcg_schedule=7)
+        assert ei.value.code == gslam.ERR_ARG
 
 
 @pytest.mark.parametrize("name", ["grid800", "park2000"])
@@ -250,15 +293,18 @@ def test_non_finite_system_fails_loudly():
         assert ei.value.code == gslam.ERR_ARG
 
 
-def test_zero_rhs_breaks_on_pq_zero_and_cap():
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_zero_rhs_breaks_on_pq_zero_and_cap(schedule):
     gc = G.grid_city(400, seed=5)
+    kw = dict(pcg_schedule=schedule)
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
         g.assemble()
-        x, st = g.solve_rhs(np.zeros(3 * 400), "pcg64", 1e-6, 100)
+        x, st = g.solve_rhs(np.zeros(3 * 400), "pcg64", 1e-6, 100, **kw)
         assert not x.any() and st["flags"] & 1 and st["iters"] == 0   # MatrixOps.c:357
-        st = g.solve("pcg64", 1e-14, 7)                                # iteration cap
-        assert st["iters"] == 7 and not st["converged"]
-        st = g.solve("pcg64", 1e-6, 0)
+        for cap in (1, 2, 7, 300):                                     # iteration cap
+            st = g.solve("pcg64", 1e-14, cap, **kw)
+            assert st["iters"] == cap and not st["converged"], cap
+        st = g.solve("pcg64", 1e-6, 0, **kw)
         assert st["iters"] == 0 and not g.dx().any()
 
 
