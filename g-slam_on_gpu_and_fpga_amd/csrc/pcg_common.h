@@ -42,7 +42,9 @@ constexpr int kFusedParts = 5;  // p.q, r.z, q.z, q.D^-1 q, r.r
 // evicting the vectors that can (FP64 SpMV at N = 1M: 109 -> 96.5 us; FP32
 // SpMV at N = 4M: 202 -> 184 us).  Smaller arrays use ordinary loads, which
 // keep them cached across iterations (N = 100k: 10.4 us against 11.1 us
-// nontemporal).  Once the FP64 solver's six vectors no longer fit either, the
+// nontemporal); the FP32 operator, which shares the cache with the mixed
+// iteration's FP64 x, FP32 vectors and D^-1, switches at half the cache.
+// Once the FP64 solver's six vectors no longer fit either, the
 // ones that only their own rows' threads read and write (x, q, r) are
 // streamed nontemporally too, which leaves the cache to z and p, whose
 // entries the SpMV gathers (FP64 iteration at N = 4M: 589 -> 561 us).

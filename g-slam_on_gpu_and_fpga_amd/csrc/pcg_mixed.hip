@@ -583,12 +583,16 @@ int pcg_solve_mixed(DeviceGraph &g, const gslam_solve_opts &o, const double *b, 
 }
 
 // The operator kernel of this graph: the nontemporal variant where its slots
-// (44 B each: factors, entry and both poses) exceed the Infinity Cache.
+// (44 B each: factors, entry and both poses) exceed half the Infinity Cache,
+// whose other half the mixed iteration's FP64 x, FP32 vectors and D^-1 fill
+// (FP32 iteration in the solve at N = 1.2M: 91.3 -> 88.4 us; at 0.8M-2M
+// within 1%; at 500k, 84 MB of slots, ordinary loads are faster: 39.3
+// against 40.9 us).
 using MixedSpmvKernel = decltype(&mixed_spmv_kernel<false>);
 
 static MixedSpmvKernel mixed_spmv_kernel_for(const DeviceGraph &g)
 {
-    return streams_past_cache(44 * g.mt_slots) ? mixed_spmv_kernel<true> : mixed_spmv_kernel<false>;
+    return streams_past_cache(2 * 44 * g.mt_slots) ? mixed_spmv_kernel<true> : mixed_spmv_kernel<false>;
 }
 
 static int mixed_phase(DeviceGraph &g, const gslam_solve_opts &o, const double *b, double *x_out,
