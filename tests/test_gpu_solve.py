@@ -140,6 +140,28 @@ def test_pcg64_one_launch_schedule_against_oracle(oracle):
         assert abs(its["auto"] - its["two_launch"]) <= 0.02 * its["two_launch"], its
 
 
+def test_nontemporal_streams_against_oracle(oracle):
+    """Grid city N = 2,000,000: the tile copy (~700 MB), the six FP64 solver
+    vectors (288 MB) and the FP32 operator's slots all exceed their
+    Infinity-Cache thresholds, so the SpMV, update and mixed kernels run
+    their nontemporal variants (same arithmetic, other load and store
+    instructions).  Fifty FP64 iterations against fifty of the CPU oracle,
+    and a mixed solve certified on its FP64 true residual."""
+    gc = G.grid_city(2000000, seed=9)
+    with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
+        g.assemble()
+        assert 76 * 256 * (g.info["n_blocks"] // 256) > 256 << 20
+        assert 144 * g.info["n_poses"] > 256 << 20
+        rp, ci, va, xi = g.system_csr()
+        st = g.solve("pcg64", 1e-30, 50)
+        assert st["iters"] == 50 and not st["converged"]
+        x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
+        assert it_o == 50
+        assert rel(g.dx(), x_o) <= 1e-9
+        st = g.solve("mixed", 1e-3, 60000)
+        assert st["converged"] and st["true_rel_res"] < 1e-3
+
+
 def test_pcg_schedule_is_validated():
     gc = G.grid_city(300, seed=4)
     with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1)) as g:
