@@ -95,15 +95,13 @@ def mixed_spmv_bytes(n_poses, n_inc):
 
 
 def assembly_bytes(n_poses, n_edges, n_blocks):
-    """Algorithmic bytes of one assembly of a between-edge graph (edge, block
-    and xi kernels): per edge its two poses, 7 parameters and the 48-double
-    staged record written (48 + 56 + 384 B); per block its contribution
-    pointer and values written (4 + 72 B) and its contributions (4 per edge:
-    index and 9 staged values, 4 + 72 B each); per pose its xi pointer and 3
-    xi values written (4 + 24 B) and its 2 contributions per incident edge
-    end (index and 3 staged values, 4 + 24 B each)."""
-    return (n_edges * (48 + 56 + 384) + n_blocks * 76 + 4 * n_edges * 76 + n_poses * 28
-            + 2 * n_edges * 28)
+    """Algorithmic bytes of one assembly of a between-edge graph, counted as
+    the minimal traffic: each edge's inputs once (its two poses, 48 B, and
+    its parameters: i, j, measurement, information and weight, 64 B), each
+    Omega block written once (72 B) and xi written once (24 B per pose).  The
+    pose walk reads each edge at both of its ends, and the staged schedule
+    writes and re-reads a 384-byte record per edge; neither is counted."""
+    return n_edges * (48 + 64) + n_blocks * 72 + n_poses * 24
 
 
 def measured_traffic():
@@ -665,14 +663,15 @@ def profile_kernel(args, gslam, G):
         return
     if args.profile_kernel == "assembly":
         gc = G.grid_city(args.n, seed=args.seed)
-        with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1)) as g:
-            g.assemble()
-            us = g.time_kernel("assembly", 20)
-            byts = assembly_bytes(g.info["n_poses"], g.info["n_edges"], g.info["n_blocks"])
-            log(f"assembly N={args.n}: {us:.2f} us per assembly (edge, block and xi kernels, 23 "
-                f"launches each), {byts / 1e6:.1f} MB algorithmic, "
-                f"{byts / (us * 1e-6) / 1e9:.0f} GB/s = "
-                f"{100 * byts / (us * 1e-6) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak")
+        for sched in ("pose_walk", "staged"):
+            with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1), assembly=sched) as g:
+                g.assemble()
+                us = g.time_kernel("assembly", 20)
+                byts = assembly_bytes(g.info["n_poses"], g.info["n_edges"], g.info["n_blocks"])
+                log(f"assembly ({sched}) N={args.n}: {us:.2f} us per assembly (23 launches), "
+                    f"{byts / 1e6:.1f} MB algorithmic, {byts / (us * 1e-6) / 1e9:.0f} GB/s = "
+                    f"{100 * byts / (us * 1e-6) / 1e9 / HBM_PEAK_GBS:.1f}% of HBM peak; "
+                    f"handle {g.device_bytes() / 1e6:.0f} MB")
         return
     if args.profile_kernel == "pcg_solve":
         # whole FP64 PCG solves (tol 1e-10) with the chosen launch schedule

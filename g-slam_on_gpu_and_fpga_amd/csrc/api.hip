@@ -182,6 +182,20 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.status_ring), 2 * sizeof(PcgStatus)));
     GSLAM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.scal_host), 16 * sizeof(double)));
 
+    // assembly schedule: the pose walk when no pose has more than
+    // kWalkMaxIncidence edge ends (its thread walks them one after another)
+    // and no edge joins a pose to itself (its four blocks would be one)
+    if (d.assembly < GSLAM_ASSEMBLY_AUTO || d.assembly > GSLAM_ASSEMBLY_STAGED) {
+        set_error("assembly must be GSLAM_ASSEMBLY_AUTO, _POSE_WALK or _STAGED");
+        return GSLAM_ERR_ARG;
+    }
+    if (d.assembly == GSLAM_ASSEMBLY_POSE_WALK && h.self_loops) {
+        set_error("GSLAM_ASSEMBLY_POSE_WALK: an edge joins a pose to itself");
+        return GSLAM_ERR_ARG;
+    }
+    g.walk = d.assembly == GSLAM_ASSEMBLY_POSE_WALK ||
+             (d.assembly == GSLAM_ASSEMBLY_AUTO && !h.self_loops && h.max_incidence <= kWalkMaxIncidence);
+
     const int64_t N = h.n, dim = 3 * N, E = h.n_e, B = h.n_blocks;
     GSLAM_TRY(g.alloc(&g.poses, dim));
     GSLAM_TRY(g.alloc(&g.dx, dim));
@@ -194,8 +208,12 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.alloc(&g.brow_ptr, N + 1));
     GSLAM_TRY(g.alloc(&g.bcol, B));
     GSLAM_TRY(g.alloc(&g.bdiag, N));
-    GSLAM_TRY(g.alloc(&g.cptr, B + 1));
-    GSLAM_TRY(g.alloc(&g.clist, 4 * E));
+    if (!g.walk) {
+        GSLAM_TRY(g.alloc(&g.cptr, B + 1));
+        GSLAM_TRY(g.alloc(&g.clist, 4 * E));
+    } else {
+        GSLAM_TRY(g.alloc(&g.xblk, 2 * E));
+    }
     GSLAM_TRY(g.alloc(&g.xptr, N + 1));
     GSLAM_TRY(g.alloc(&g.xlist, 2 * E));
     GSLAM_TRY(g.alloc(&g.csr_row_ptr, dim + 1));
@@ -204,7 +222,7 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     g.n_tiles = static_cast<int32_t>(h.tile_row.size()) - 1;
     GSLAM_TRY(g.alloc(&g.tile_info, g.n_tiles));
     GSLAM_TRY(g.alloc(&g.tcol, static_cast<int64_t>(kTileBlocks) * g.n_tiles));
-    GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
+    if (!g.walk) GSLAM_TRY(g.alloc(&g.stage, static_cast<int64_t>(kStage) * E));
     GSLAM_TRY(g.alloc(&g.bval, 9 * B));
     GSLAM_TRY(g.alloc(&g.xi, dim));
     GSLAM_TRY(g.alloc(&g.x, dim));
@@ -231,8 +249,12 @@ static int create_device(const gslam_graph_desc &d, DeviceGraph &g)
     GSLAM_TRY(g.upload(g.brow_ptr, h.brow_ptr.data(), (N + 1) * 4));
     GSLAM_TRY(g.upload(g.bcol, h.bcol.data(), B * 4));
     GSLAM_TRY(g.upload(g.bdiag, h.bdiag.data(), N * 4));
-    GSLAM_TRY(g.upload(g.cptr, h.cptr.data(), (B + 1) * 4));
-    GSLAM_TRY(g.upload(g.clist, h.clist.data(), 4 * E * 4));
+    if (!g.walk) {
+        GSLAM_TRY(g.upload(g.cptr, h.cptr.data(), (B + 1) * 4));
+        GSLAM_TRY(g.upload(g.clist, h.clist.data(), 4 * E * 4));
+    } else {
+        GSLAM_TRY(g.upload(g.xblk, h.xblk.data(), 2 * E * 4));
+    }
     GSLAM_TRY(g.upload(g.xptr, h.xptr.data(), (N + 1) * 4));
     GSLAM_TRY(g.upload(g.xlist, h.xlist.data(), 2 * E * 4));
     GSLAM_TRY(g.upload(g.csr_row_ptr, h.csr_row_ptr.data(), (dim + 1) * 4));

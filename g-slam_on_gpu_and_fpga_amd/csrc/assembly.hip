@@ -1,16 +1,19 @@
 // Edge evaluation and deterministic assembly of Omega (BSR 3x3) and xi.
 //
-// Two passes, both HBM-streaming:
-//   1. one thread per edge evaluates the edge model at the current poses and
-//      writes its 42 values (4 blocks + 2 xi halves) to a 48-double stage
-//      record;
-//   2. one thread per Omega value (9 per block) and per xi entry sums its
-//      contribution list, which lists stage records in edge order, starting
-//      from the anchor.  Every entry is therefore summed in the order of
-//      build_csr_from_edges (src/cpu_pcg_fp64/MatrixOps.c:223-272), with no
-//      atomics: Omega is bitwise repeatable run to run, unlike the
-//      reference's atomicAdd scatter (src/gpu_pcg_fp64/device_graph.cu:437-456,
-//      805-826), and no per-round binary searches are needed.
+// Every entry is summed in edge order, starting from the anchor -- the order
+// of build_csr_from_edges (src/cpu_pcg_fp64/MatrixOps.c:223-272) -- with no
+// atomics: Omega is bitwise repeatable run to run, unlike the reference's
+// atomicAdd scatter (src/gpu_pcg_fp64/device_graph.cu:437-456, 805-826), and
+// no per-round binary searches are needed.  Two schedules give the same
+// values:
+//   pose walk (graphs where no pose has more than kWalkMaxIncidence edge
+//     ends): one thread per pose evaluates each incident edge in edge order
+//     and adds its diagonal block, its off-diagonal block (a precomputed
+//     incidence -> block map; the first contribution of a block stores) and
+//     its xi half -- one pass, each edge evaluated once per end;
+//   staged: one thread per edge writes its 42 values (4 blocks + 2 xi
+//     halves) to a 48-double stage record, then one thread per Omega value
+//     (9 per block) and per xi entry sums its contribution list.
 #include "device_common.h"
 #include "gslam_internal.h"
 
@@ -19,18 +22,34 @@ namespace gslam {
 // 8 FP32 factors of one edge, stored at the slots of both of its incidence
 // entries (the mixed solver's factorized operator, pcg_mixed.hip): factor c
 // of slot 256 h + l at frec[(8 h + c) 256 + l].
+__device__ inline void put_factor_slot(float *__restrict__ frec, int64_t s, float4 a, float4 b)
+{
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    float *f = frec + (s / kTileBlocks) * 8 * kTileBlocks + s % kTileBlocks;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) f[c * kTileBlocks] = v[c];
+}
+
 __device__ inline void put_factors(float *__restrict__ frec, const int32_t *__restrict__ mt_pos,
                                    int64_t e, float4 a, float4 b)
 {
     if (!frec) return;
-    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const int64_t s = mt_pos[2 * e + side];
-        float *f = frec + (s / kTileBlocks) * 8 * kTileBlocks + s % kTileBlocks;
+    for (int side = 0; side < 2; ++side) put_factor_slot(frec, mt_pos[2 * e + side], a, b);
+}
+
+// One edge evaluated at the current poses: its four 3x3 blocks UL (t1,t1),
+// UR (t1,t2), BL (t2,t1), BR (t2,t2) row-major, then the xi halves of t1 and
+// t2, and the 8 FP32 factors of the mixed operator.
+struct EdgeRec {
+    double s[42];
+    float4 fa, fb;
+};
+
+__device__ inline void store_record(double *__restrict__ st, const EdgeRec &R)
+{
 #pragma unroll
-        for (int c = 0; c < 8; ++c) f[c * kTileBlocks] = v[c];
-    }
+    for (int k = 0; k < 42; ++k) st[k] = R.s[k];
 }
 
 // Velocity-model motion edge (i, i+1).  Model: src/cpu_pcg_fp64/MotionEdge.c
@@ -38,15 +57,11 @@ __device__ inline void put_factors(float *__restrict__ frec, const int32_t *__re
 // 1e-4 I)^-1, blocks lambda*[F^T Om F, -F^T Om, -Om F, Om], xi1 = lambda F^T
 // Om e, xi2 = -lambda Om e, e = x2 - x_pred (theta wrapped only in wrap
 // mode).  The 3x3 inverse is the symmetric cofactor form.
-__global__ void __launch_bounds__(kThreads)
-motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__ mp,
-                    int64_t n_mot, int64_t e_off, double delta, double delta_t,
-                    double lambda, double m0, double m1, double m2, double m3,
-                    int wrap, double *__restrict__ stage, float *__restrict__ frec,
-                    const int32_t *__restrict__ mt_pos)
+__device__ inline void motion_record(const double *__restrict__ poses, const double *__restrict__ mp,
+                                     int64_t i, double delta, double delta_t, double lambda,
+                                     double m0, double m1, double m2, double m3, int wrap,
+                                     EdgeRec &R)
 {
-    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (i >= n_mot) return;
     const double x10 = poses[3 * i], x11 = poses[3 * i + 1], th = poses[3 * i + 2];
     const double x20 = poses[3 * i + 3], x21 = poses[3 * i + 4], x22 = poses[3 * i + 5];
     const double nu = mp[2 * i];
@@ -85,7 +100,7 @@ motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__
         T[3 + j] = Om[3 + j];
         T[6 + j] = F02 * Om[j] + F12 * Om[3 + j] + Om[6 + j];
     }
-    double *s = stage + (e_off + i) * kStage;
+    double *s = R.s;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
         s[0 + 3 * r + 0] = lambda * T[3 * r];
@@ -115,26 +130,35 @@ motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__
         s[39 + r] = -lambda * (Om[3 * r] * e0 + Om[3 * r + 1] * e1 + Om[3 * r + 2] * e2);
     }
     // factors: J1 = -F, J2 = I, W = lambda Om
-    put_factors(frec, mt_pos, e_off + i,
-                make_float4(static_cast<float>(F02), static_cast<float>(F12),
-                            static_cast<float>(lambda * Om[0]), static_cast<float>(lambda * Om[1])),
-                make_float4(static_cast<float>(lambda * Om[2]), static_cast<float>(lambda * Om[4]),
-                            static_cast<float>(lambda * Om[5]), static_cast<float>(lambda * Om[8])));
+    R.fa = make_float4(static_cast<float>(F02), static_cast<float>(F12),
+                       static_cast<float>(lambda * Om[0]), static_cast<float>(lambda * Om[1]));
+    R.fb = make_float4(static_cast<float>(lambda * Om[2]), static_cast<float>(lambda * Om[4]),
+                       static_cast<float>(lambda * Om[5]), static_cast<float>(lambda * Om[8]));
+}
+
+__global__ void __launch_bounds__(kThreads)
+motion_edges_kernel(const double *__restrict__ poses, const double *__restrict__ mp,
+                    int64_t n_mot, int64_t e_off, double delta, double delta_t,
+                    double lambda, double m0, double m1, double m2, double m3,
+                    int wrap, double *__restrict__ stage, float *__restrict__ frec,
+                    const int32_t *__restrict__ mt_pos)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n_mot) return;
+    EdgeRec R;
+    motion_record(poses, mp, i, delta, delta_t, lambda, m0, m1, m2, m3, wrap, R);
+    store_record(stage + (e_off + i) * kStage, R);
+    put_factors(frec, mt_pos, e_off + i, R.fa, R.fb);
 }
 
 // Landmark-pair observation edge.  Model: src/cpu_pcg_fp64/ObsEdge.c:10-212:
 // e = x2 - x1 + r2 (c2, s2) - r1 (c1, s1), Sigma = R1 Q1 R1^T + R2 Q2 R2^T,
 // Om = Sigma^-1, B1 = [-I | b1], B2 = [I | b2], blocks Bi^T Om Bj,
 // xi = -B^T Om e.  No lambda (ObsEdge.c:209).
-__global__ void __launch_bounds__(kThreads)
-obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
-                 const int32_t *__restrict__ t2, const double *__restrict__ op,
-                 int64_t n_obs_e, double snr0, double snr1, double *__restrict__ stage,
-                 float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
+__device__ inline void obs_record(const double *__restrict__ poses, int64_t a, int64_t b,
+                                  const double *__restrict__ op, int64_t e, double snr0,
+                                  double snr1, EdgeRec &R)
 {
-    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (e >= n_obs_e) return;
-    const int64_t a = t1[e], b = t2[e];
     const double r1 = op[4 * e], p1 = op[4 * e + 1], r2 = op[4 * e + 2], p2 = op[4 * e + 3];
     double s1, c1, s2, c2;
     sincos(poses[3 * a + 2] + p1, &s1, &c1);
@@ -155,7 +179,7 @@ obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t
     const double Ob2x = o00 * b2x + o01 * b2y, Ob2y = o01 * b2x + o11 * b2y;  // Om b2
     const double b1Ob1 = b1x * Ob1x + b1y * Ob1y, b1Ob2 = b1x * Ob2x + b1y * Ob2y;
     const double b2Ob1 = b2x * Ob1x + b2y * Ob1y, b2Ob2 = b2x * Ob2x + b2y * Ob2y;
-    double *s = stage + e * kStage;
+    double *s = R.s;
     // UL = B1^T Om B1 = [[Om, -Om b1], [-b1^T Om, b1^T Om b1]]
     s[0] = o00;   s[1] = o01;   s[2] = -Ob1x;
     s[3] = o01;   s[4] = o11;   s[5] = -Ob1y;
@@ -180,27 +204,32 @@ obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t
     s[40] = -Oey;
     s[41] = -(b2x * Oex + b2y * Oey);
     // factors: B1 = [-I | b1], B2 = [I | b2], W = Om
-    put_factors(frec, mt_pos, e,
-                make_float4(static_cast<float>(b1x), static_cast<float>(b1y),
-                            static_cast<float>(b2x), static_cast<float>(b2y)),
-                make_float4(static_cast<float>(o00), static_cast<float>(o01),
-                            static_cast<float>(o11), 0.f));
+    R.fa = make_float4(static_cast<float>(b1x), static_cast<float>(b1y), static_cast<float>(b2x),
+                       static_cast<float>(b2y));
+    R.fb = make_float4(static_cast<float>(o00), static_cast<float>(o01), static_cast<float>(o11), 0.f);
+}
+
+__global__ void __launch_bounds__(kThreads)
+obs_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
+                 const int32_t *__restrict__ t2, const double *__restrict__ op,
+                 int64_t n_obs_e, double snr0, double snr1, double *__restrict__ stage,
+                 float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
+{
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= n_obs_e) return;
+    EdgeRec R;
+    obs_record(poses, t1[e], t2[e], op, e, snr0, snr1, R);
+    store_record(stage + e * kStage, R);
+    put_factors(frec, mt_pos, e, R.fa, R.fb);
 }
 
 // SE(2) between edge (no reference counterpart):
 // e = [R(th_i)^T (t_j - t_i) - z_t ; th_j - th_i - z_th], L = w diag(info),
 // blocks Ji^T L Jj..., xi = -J^T L e.
-__global__ void __launch_bounds__(kThreads)
-between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
-                     const int32_t *__restrict__ t2, const double *__restrict__ bp,
-                     const double *__restrict__ bw, int64_t n_bet, int64_t e_off,
-                     double lambda, int wrap, int exact, double *__restrict__ stage,
-                     float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
+__device__ inline void between_record(const double *__restrict__ poses, int64_t a, int64_t b,
+                                      const double *__restrict__ bp, const double *__restrict__ bw,
+                                      int64_t k, double lambda, int wrap, int exact, EdgeRec &R)
 {
-    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (k >= n_bet) return;
-    const int64_t e = e_off + k;
-    const int64_t a = t1[e], b = t2[e];
     double s, c;
     sincos(poses[3 * a + 2], &s, &c);
     const double dx = poses[3 * b] - poses[3 * a], dy = poses[3 * b + 1] - poses[3 * a + 1];
@@ -216,7 +245,7 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
     const double Jj[9] = {c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0};
     const double L[3] = {L0, L1, L2};
     const double ev[3] = {e0, e1, e2};
-    double *st = stage + e * kStage;
+    double *st = R.s;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const double *JA = (q < 2) ? Ji : Jj;
@@ -252,10 +281,87 @@ between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict
         st[18 + 5] -= cy;
     }
     // factors: rotation (c, s), theta column (ga, gb), W = diag(L)
-    put_factors(frec, mt_pos, e,
-                make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(ga),
-                            static_cast<float>(gb)),
-                make_float4(static_cast<float>(L0), static_cast<float>(L1), static_cast<float>(L2), 0.f));
+    R.fa = make_float4(static_cast<float>(c), static_cast<float>(s), static_cast<float>(ga),
+                       static_cast<float>(gb));
+    R.fb = make_float4(static_cast<float>(L0), static_cast<float>(L1), static_cast<float>(L2), 0.f);
+}
+
+__global__ void __launch_bounds__(kThreads)
+between_edges_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
+                     const int32_t *__restrict__ t2, const double *__restrict__ bp,
+                     const double *__restrict__ bw, int64_t n_bet, int64_t e_off,
+                     double lambda, int wrap, int exact, double *__restrict__ stage,
+                     float *__restrict__ frec, const int32_t *__restrict__ mt_pos)
+{
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= n_bet) return;
+    const int64_t e = e_off + k;
+    EdgeRec R;
+    between_record(poses, t1[e], t2[e], bp, bw, k, lambda, wrap, exact, R);
+    store_record(stage + e * kStage, R);
+    put_factors(frec, mt_pos, e, R.fa, R.fb);
+}
+
+// Pose walk: one thread per pose i.  Its incidence entries (e*2+side, in
+// edge order) give, per edge, the diagonal contribution (UL for side 0, BR
+// for side 1) -- summed in registers after the anchor (MatrixOps.c:223-231)
+// --, the off-diagonal one (UR into block (t1,t2), BL into (t2,t1): block
+// row i, which only this thread writes; xblk holds -(b+1) for a block's first
+// contribution, which stores 0.0 + v, as the staged sum starts from 0.0) and
+// the xi half; the edge's FP32 factors go to this end's operator slot.
+__global__ void __launch_bounds__(kThreads)
+assemble_walk_kernel(const double *__restrict__ poses, const int32_t *__restrict__ t1,
+                     const int32_t *__restrict__ t2, const double *__restrict__ op,
+                     const double *__restrict__ mp, const double *__restrict__ bp,
+                     const double *__restrict__ bw, int64_t n_obs_e, int64_t n_mot_e,
+                     double snr0, double snr1, double delta, double delta_t, double lambda,
+                     double m0, double m1, double m2, double m3, int wrap, int exact,
+                     const int32_t *__restrict__ xptr, const int32_t *__restrict__ xlist,
+                     const int32_t *__restrict__ xblk, const int32_t *__restrict__ bdiag,
+                     int32_t n, double anchor, int prior, double ref0, double ref1, double ref2,
+                     double *__restrict__ bval, double *__restrict__ xi, float *__restrict__ frec,
+                     const int32_t *__restrict__ mt_pos)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double dg[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dg[k] = (i == 0 && (k == 0 || k == 4 || k == 8)) ? anchor : 0.0;
+    double xv[3] = {0.0, 0.0, 0.0};
+    const int32_t c1 = xptr[i + 1];
+    for (int32_t c = xptr[i]; c < c1; ++c) {
+        const int32_t ent = xlist[c];
+        const int64_t e = ent >> 1;
+        const bool side = ent & 1;
+        EdgeRec R;
+        if (e < n_obs_e)
+            obs_record(poses, t1[e], t2[e], op, e, snr0, snr1, R);
+        else if (e < n_obs_e + n_mot_e)
+            motion_record(poses, mp, e - n_obs_e, delta, delta_t, lambda, m0, m1, m2, m3, wrap, R);
+        else
+            between_record(poses, t1[e], t2[e], bp, bw, e - n_obs_e - n_mot_e, lambda, wrap, exact,
+                           R);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dg[k] += side ? R.s[27 + k] : R.s[k];
+        const int32_t xb = xblk[ent];
+        double *bv = bval + 9 * static_cast<int64_t>(xb < 0 ? -(xb + 1) : xb);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) bv[k] = (xb < 0 ? 0.0 : bv[k]) + (side ? R.s[18 + k] : R.s[9 + k]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) xv[r] += side ? R.s[39 + r] : R.s[36 + r];
+        if (frec) put_factor_slot(frec, mt_pos[ent], R.fa, R.fb);
+    }
+    double *bd = bval + 9 * static_cast<int64_t>(bdiag[i]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) bd[k] = dg[k];
+    const int64_t o = 3 * static_cast<int64_t>(i);
+    const double ref[3] = {ref0, ref1, ref2};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double v = xv[r];
+        if (prior && i == 0) v += anchor * (ref[r] - poses[r]);
+        xi[o + r] = v;
+    }
 }
 
 // Omega values: one thread per (block, element); contributions in edge order
@@ -325,12 +431,23 @@ static inline unsigned int nblk(int64_t n) { return static_cast<unsigned int>((n
 int launch_assembly(DeviceGraph &g, bool exact_hessian)
 {
     const HostGraph &h = g.h;
+    const double delta_t = static_cast<double>(static_cast<unsigned int>(g.delta));
+    if (g.walk) {
+        hipLaunchKernelGGL(assemble_walk_kernel, dim3(nblk(h.n)), dim3(kThreads), 0, g.stream,
+                           g.poses, g.e_t1, g.e_t2, g.obs_param, g.mot_param, g.bet_param, g.bet_w,
+                           h.n_obs_e, h.n_mot_e, g.snr[0], g.snr[1], g.delta, delta_t, g.lambda,
+                           g.mns[0], g.mns[1], g.mns[2], g.mns[3], g.wrap, exact_hessian ? 1 : 0,
+                           g.xptr, g.xlist, g.xblk, g.bdiag, h.n, g.anchor, g.anchor_prior,
+                           g.pose0_ref[0], g.pose0_ref[1], g.pose0_ref[2], g.bval, g.xi, g.frec,
+                           g.mt_pos);
+        GSLAM_HIP(hipGetLastError());
+        return GSLAM_OK;
+    }
     if (h.n_obs_e > 0)
         hipLaunchKernelGGL(obs_edges_kernel, dim3(nblk(h.n_obs_e)), dim3(kThreads), 0, g.stream,
                            g.poses, g.e_t1, g.e_t2, g.obs_param, h.n_obs_e, g.snr[0], g.snr[1],
                            g.stage, g.frec, g.mt_pos);
     if (h.n_mot_e > 0) {
-        const double delta_t = static_cast<double>(static_cast<unsigned int>(g.delta));
         hipLaunchKernelGGL(motion_edges_kernel, dim3(nblk(h.n_mot_e)), dim3(kThreads), 0,
                            g.stream, g.poses, g.mot_param, h.n_mot_e, h.n_obs_e, g.delta,
                            delta_t, g.lambda, g.mns[0], g.mns[1], g.mns[2], g.mns[3], g.wrap,

@@ -217,11 +217,20 @@ int build_host_graph(const gslam_graph_desc &d, HostGraph &g, std::string &err)
             for (int q = 0; q < 4; ++q)
                 g.clist[f[eslot[4 * e + q]]++] = static_cast<int32_t>(4 * e + q);
     }
+    g.xblk.resize(static_cast<size_t>(2 * g.n_e));
+    for (int64_t e = 0; e < g.n_e; ++e) g.self_loops = g.self_loops || g.t1[e] == g.t2[e];
+    for (int64_t e = 0; e < g.n_e; ++e)
+        for (int side = 0; side < 2; ++side) {
+            const int32_t b = eslot[4 * e + 1 + side];
+            const bool first = g.clist[g.cptr[b]] == static_cast<int32_t>(4 * e + 1 + side);
+            g.xblk[2 * e + side] = first ? -(b + 1) : b;
+        }
     g.xptr.assign(static_cast<size_t>(N) + 1, 0);
     for (int64_t e = 0; e < g.n_e; ++e) {
         g.xptr[g.t1[e] + 1]++;
         g.xptr[g.t2[e] + 1]++;
     }
+    for (int32_t i = 0; i < N; ++i) g.max_incidence = std::max(g.max_incidence, g.xptr[i + 1]);
     for (int32_t i = 0; i < N; ++i) g.xptr[i + 1] += g.xptr[i];
     g.xlist.resize(static_cast<size_t>(2 * g.n_e));
     {

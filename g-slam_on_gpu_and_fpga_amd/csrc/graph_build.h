@@ -21,6 +21,8 @@ constexpr int kStage = 48;
 // workgroup in 9 passes) and 85 block rows (3 outputs per row <= 256 threads).
 constexpr int kTileBlocks = 256;
 constexpr int kTileRows = 85;
+// Assembly by pose walk (assembly.hip) when no pose has more edge ends.
+constexpr int kWalkMaxIncidence = 64;
 
 struct HostGraph {
     int32_t n = 0;                 // poses
@@ -41,6 +43,13 @@ struct HostGraph {
     // (entries e*4+q in edge order); pose i receives xi halves
     // xlist[xptr[i]..xptr[i+1]) (entries e*2+side in edge order).
     std::vector<int32_t> cptr, clist, xptr, xlist;
+    // Per incidence entry e*2+side: the off-diagonal block it adds to (UR of
+    // (t1,t2) for side 0, BL of (t2,t1) for side 1), stored as -(b+1) when it
+    // is that block's first contribution in edge order; and the largest
+    // number of incidence entries of one pose.
+    std::vector<int32_t> xblk;
+    int32_t max_incidence = 0;
+    bool self_loops = false;  // an edge with t1 == t2 (its four blocks are one)
 
     // SpMV tiles: tile t covers block rows [tile_row[t], tile_row[t+1]),
     // whole rows, at most kTileBlocks blocks and kTileRows rows unless a

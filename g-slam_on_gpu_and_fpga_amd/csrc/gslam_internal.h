@@ -95,6 +95,11 @@ struct DeviceGraph {
     // structure
     int32_t *brow_ptr = nullptr, *bcol = nullptr, *bdiag = nullptr;
     int32_t *cptr = nullptr, *clist = nullptr, *xptr = nullptr, *xlist = nullptr;
+    // assembly schedule (assembly.hip): the pose walk reads xblk (per
+    // incidence entry its off-diagonal block); the staged one reads cptr,
+    // clist and the stage records
+    bool walk = false;
+    int32_t *xblk = nullptr;
     int32_t *csr_row_ptr = nullptr, *csr_col_idx = nullptr, *csr_pos = nullptr;
     int4 *tile_info = nullptr;    // per tile {first row, end row, first block, end block}
     int32_t n_tiles = 0;

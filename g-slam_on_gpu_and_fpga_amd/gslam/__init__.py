@@ -46,7 +46,11 @@ class _Desc(C.Structure):
         ("snr", C.c_double * 2),
         ("n_between", C.c_int64), ("be_i", C.c_void_p), ("be_j", C.c_void_p),
         ("be_meas", C.c_void_p), ("be_info", C.c_void_p),
-        ("anchor", C.c_double), ("wrap", C.c_int32), ("anchor_prior", C.c_int32)]
+        ("anchor", C.c_double), ("wrap", C.c_int32), ("anchor_prior", C.c_int32),
+        ("assembly", C.c_int32)]
+
+
+ASSEMBLY = {"auto": 0, "pose_walk": 1, "staged": 2}
 
 
 class _Info(C.Structure):
@@ -275,7 +279,7 @@ class Graph:
 
     def __init__(self, poses, *, motion=False, u_nu=None, u_omega=None, delta=1.0, lam=0.1,
                  mns=(0.19, 0.001, 0.13, 0.2), obs=None, pair_order=0, snr=(0.14, 0.05),
-                 between=None, anchor=1e6, wrap=0, anchor_prior=0):
+                 between=None, anchor=1e6, wrap=0, anchor_prior=0, assembly="auto"):
         L = lib()
         poses = _f64(poses)
         n = poses.size // 3
@@ -311,6 +315,7 @@ class Graph:
         d.anchor = anchor
         d.wrap = wrap
         d.anchor_prior = anchor_prior
+        d.assembly = ASSEMBLY[assembly] if isinstance(assembly, str) else int(assembly)
         h = C.c_void_p()
         _check(L.gslam_create(C.byref(d), C.byref(h)), "gslam_create")
         self._keep = []
@@ -329,13 +334,15 @@ class Graph:
 
     @classmethod
     def from_log(cls, path: str | None = None, lam: float = 0.1, pair_order: int = 0,
-                 wrap: int = 0, data: dict | None = None, anchor_prior: int = 0):
+                 wrap: int = 0, data: dict | None = None, anchor_prior: int = 0,
+                 assembly="auto"):
         g = read_log(path) if data is None else data
         return cls(g["poses"], motion=True, u_nu=g["u_nu"], u_omega=g["u_omega"],
                    delta=g["delta"], lam=lam,
                    obs=dict(z_step=g["z_step"], z_lm=g["z_lm"], z_range=g["z_range"],
                             z_bearing=g["z_bearing"]),
-                   pair_order=pair_order, wrap=wrap, anchor_prior=anchor_prior)
+                   pair_order=pair_order, wrap=wrap, anchor_prior=anchor_prior,
+                   assembly=assembly)
 
     # -------------------------------------------------------------- lifecycle
     def close(self):

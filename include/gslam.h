@@ -87,7 +87,19 @@ typedef struct {
                                    gslam_create, adding anchor * (ref - pose0)
                                    to xi, so the converged poses are a smooth
                                    function of the edge weights (gslam_adjoint) */
+    int32_t assembly;           /* GSLAM_ASSEMBLY_AUTO, _POSE_WALK or _STAGED */
 } gslam_graph_desc;
+
+/* Assembly of Omega and xi (the same values, summed in the same edge order):
+ *   POSE_WALK: one thread per pose evaluates each incident edge and adds its
+ *              blocks and xi half in edge order (no staging pass);
+ *   STAGED:    one thread per edge stages its 4 blocks and 2 xi halves, then
+ *              one thread per Omega / xi entry sums its contribution list;
+ *   AUTO:      POSE_WALK when no pose has more than 64 incident edge ends and
+ *              no edge joins a pose to itself, else STAGED. */
+#define GSLAM_ASSEMBLY_AUTO      0
+#define GSLAM_ASSEMBLY_POSE_WALK 1
+#define GSLAM_ASSEMBLY_STAGED    2
 
 typedef struct {
     int32_t n_poses;

@@ -135,3 +135,66 @@ def test_assembly_is_bitwise_repeatable():
             g.assemble()
             b = g.system_bsr()
             assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("case", ["grid", "park_prior", "ct100", "vp100"])
+def test_pose_walk_and_staged_assembly_agree_bitwise(case):
+    """The two assembly schedules (one thread per pose walking its edges;
+    one thread per edge staging its record, then one per entry summing) add
+    the same edge records in the same edge order, so Omega and xi agree bit
+    for bit -- and so do the mixed operator's FP32 factors (through forty
+    mixed PCG iterations) and, with the anchor prior, the exact-Hessian
+    assembly of the adjoint (through its gradients)."""
+    if case == "grid":
+        gc = G.grid_city(2000, seed=3)
+        make = lambda a: gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1, wrap=1), assembly=a)
+    elif case == "park_prior":
+        gc = G.park(1500, seed=2, route="laps")
+        make = lambda a: gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 1.0, wrap=1),
+                                     anchor_prior=1, assembly=a)
+    else:
+        path = data_path(GOLDEN_SETS[case])
+        make = lambda a: gslam.Graph.from_log(path, lam=0.1, assembly=a)
+    with make("pose_walk") as gw, make("staged") as gs:
+        for g in (gw, gs):
+            g.assemble()
+        assert all(np.array_equal(x, y) for x, y in zip(gw.system_bsr(), gs.system_bsr()))
+        dx = []
+        for g in (gw, gs):
+            st = g.solve("mixed", 1e-30, 40)
+            assert st["iters"] == 40
+            dx.append(g.dx())
+        assert np.array_equal(dx[0], dx[1])
+        if case == "park_prior":
+            rng = np.random.default_rng(1)
+            dl = rng.normal(size=3 * gw.n)
+            for g in (gw, gs):
+                g.gn_step("pcg64", 1e-10, 100000)
+            aw, a_s = gw.adjoint(dl, tol=1e-10), gs.adjoint(dl, tol=1e-10)
+            assert np.array_equal(aw[0], a_s[0]) and aw[1] == a_s[1]
+
+
+def test_assembly_schedule_choice():
+    """'auto' walks poses unless a pose has more than 64 edge ends or an edge
+    joins a pose to itself; forcing the walk on such a self pair is refused."""
+    poses = np.zeros(3 * 4)
+    be = dict(i=np.array([1, 1], np.int32), j=np.array([1, 2], np.int32),
+              meas=np.zeros(6), info=np.ones(6))
+    with pytest.raises(gslam.GSlamError) as ei:
+        gslam.Graph(poses, between=be, assembly="pose_walk")
+    assert ei.value.code == gslam.ERR_ARG
+    with pytest.raises(gslam.GSlamError) as ei:
+        gslam.Graph(poses, between=be, assembly=9)
+    assert ei.value.code == gslam.ERR_ARG
+    # a hub pose with 80 loop closures: 'auto' stages, and both schedules agree
+    n = 90
+    rng = np.random.default_rng(2)
+    hub = dict(i=np.r_[np.arange(n - 1), np.zeros(80, np.int64)].astype(np.int32),
+               j=np.r_[np.arange(1, n), np.arange(10, 90)].astype(np.int32),
+               meas=rng.normal(size=3 * (n - 1 + 80)), info=np.full(3 * (n - 1 + 80), 4.0))
+    p = rng.normal(size=3 * n)
+    with gslam.Graph(p, between=hub, lam=0.5, wrap=1) as ga, \
+            gslam.Graph(p, between=hub, lam=0.5, wrap=1, assembly="pose_walk") as gw:
+        ga.assemble()
+        gw.assemble()
+        assert all(np.array_equal(x, y) for x, y in zip(ga.system_bsr(), gw.system_bsr()))
