@@ -5,7 +5,8 @@
 // x += alpha p, r -= alpha q, stop when ||r|| < tol ||b|| (checked after the
 // update), z = D^-1 r, beta = rz_new/rz_old, p = z + beta p.
 //
-// Two kernels per iteration, no host round trip and no atomics:
+// Two kernels per iteration (one on graphs of at most kFusedMaxTiles SpMV
+// tiles: pcg_fused_kernel below), no host round trip and no atomics:
 //   A(k) "spmv":   sums the r.r / r.z partials B(k-1) left behind (every
 //                  workgroup redundantly, in the same fixed order, so all
 //                  agree bit for bit), applies the stopping test, forms
