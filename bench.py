@@ -12,18 +12,21 @@ N = 100,000 poses, seed 0, lambda 0.1, angle wrap on.  Accuracy: relative dx
 error against an independent direct solve of the same system (SuperLU,
 refined on residuals in x87 extended precision: ~1e-11 relative; the FP64
 LU solve alone is 1.3e-7 away), with the FP64 PCG solve at tolerance 1e-14
-reported beside it.  The FP64 solver runs at tolerance 3e-13, the loosest
-that meets rel dx err <= 1e-8 on this graph (1e-12 gives 1.3e-8); the mixed
-solver runs FP32 iterations until the FP64 true residual is below 1e-4 and
-then FP64 PCG on the correction to the same 3e-13 (FP32 iterations alone
-stall at ~4e-7 error: kappa ~ 1e13 of the block-Jacobi preconditioned
-operator is far beyond 1/eps32).
+reported beside it.  The FP64 solver runs at tolerance 3e-13 (3.3e-9 rel dx
+err on this graph; 1e-12 gives ~1.3e-8); the mixed solver runs FP32
+iterations until the FP64 true residual is below 1e-3 and then FP64 PCG on
+the correction to the same 3e-13 (7.9e-9; from 1e-4 the correction ends at
+1.1-1.4e-8 -- at kappa ~ 1e13 the attainable error is ~1e-8 within a factor
+of three, set by the rounding path -- and FP32 iterations alone stall at
+~1e-6 error, kappa far beyond 1/eps32).
 
 Also reported:
-* roofline: the FP64 BSR SpMV's mean per-launch duration inside the timed
-  solves (HIP events around sampled launches on the solver stream), its
-  algorithmic bytes and the HBM fraction; HBM traffic per launch from the
-  committed rocprofv3 counter summary when present;
+* roofline: the dominant FP64 PCG kernel's mean per-launch duration inside
+  the timed solves (HIP events around sampled launches on the solver
+  stream) -- at N=100k the one-launch iteration pcg_fused_kernel, else the
+  SpMV of the two-launch schedule --, its algorithmic bytes and the HBM
+  fraction; HBM traffic per launch from the committed rocprofv3 counter
+  summary when present;
 * cpu_baseline: the CPU oracle (FP64, one thread) on a bounded sample of the
   same GN step, extrapolated to the FP64 solve's iteration count; the same
   with the PCG loops on all OpenMP threads (omp); and, as
@@ -63,6 +66,7 @@ HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK = 157.3   # TFLOP/s, MI355X dense FP32 matrix
 FP64_PEAK = 78.6         # TFLOP/s, MI355X FP64 (vector and matrix)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_spmv_grid100k_traffic.md")
+FUSED_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_fused_grid100k_traffic.md")
 
 
 def log(msg):
@@ -75,6 +79,16 @@ def spmv_bytes(n_poses, n_blocks):
     z, p_old and x read once (3 x 24 B per row) and p_new, q and x (the
     previous iteration's x += alpha p_old) written once (3 x 24 B per row)."""
     return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 6 * 24 * n_poses
+
+
+def fused_bytes(n_poses, n_blocks):
+    """Algorithmic bytes of one launch of the one-launch FP64 PCG iteration
+    (pcg_fused_kernel): the BSR values and column indices (9*8 + 4 B per
+    block) and block-row pointers (4 B per row); per pose its record
+    {z, w = D^-1 q, p} (72 B, the gathers of neighbouring rows hitting the
+    caches), r, q and x (72 B) and its 3x3 block of D^-1 (72 B) read once,
+    and r, q, x (72 B) and the new record (72 B) written once."""
+    return n_blocks * (9 * 8 + 4) + 4 * (n_poses + 1) + 5 * 72 * n_poses
 
 
 def update_bytes(n_poses):
@@ -104,11 +118,12 @@ def assembly_bytes(n_poses, n_edges, n_blocks):
     return n_edges * (48 + 64) + n_blocks * 72 + n_poses * 24
 
 
-def measured_traffic():
-    """HBM bytes per SpMV launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+def measured_traffic(fused=False):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(FUSED_TRAFFIC_FILE if fused else TRAFFIC_FILE) as f:
             m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
@@ -203,34 +218,58 @@ def pcg_section(args, gslam, G, np):
             t_total += time.perf_counter() - t0
             iters.append(st["iters"])
             if solver == "pcg64" and st["spmv_us"] > 0:
-                samples.append((st["spmv_us"], st["update_us"]))
+                # (HIP events of the solve: its launch loop per launch -- the
+                # iterations plus the first launch --, and the sampled launches)
+                samples.append((st["spmv_us"], st["update_us"],
+                                1000.0 * (st["ms"] - st["ms_setup"]) / (st["iters"] + 1)))
         ms = 1000.0 * t_total / args.steps
         results[solver] = dict(ms_per_step=ms, tol=tol, pcg_iters=int(np.mean(iters)),
                                us_per_iter=1000.0 * ms / max(np.mean(iters), 1),
                                handle_device_mb=g.device_bytes() / 1e6, **acc)
         log(f"{solver}: {ms:.1f} ms per GN step, {np.mean(iters):.0f} PCG iterations")
 
-    # dominant kernel: FP64 SpMV, mean duration inside the timed solves
-    byts = spmv_bytes(info["n_poses"], info["n_blocks"])
+    # dominant kernel, mean duration inside the timed solves: the one-launch
+    # iteration (graphs of at most 2048 SpMV tiles; the sampled update time is
+    # then zero) or the FP64 SpMV of the two-launch schedule
+    fused = bool(samples) and all(s[1] == 0.0 for s in samples)
+    if fused:
+        byts = fused_bytes(info["n_poses"], info["n_blocks"])
+        kname = ("pcg_fused_kernel<4> (one FP64 PCG iteration: SpMV, r/x/z/w update and "
+                 "partials; four SpMV tiles per 1024-thread workgroup)")
+    else:
+        byts = spmv_bytes(info["n_poses"], info["n_blocks"])
+        kname = "pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)"
     roofline = None
     if samples:
-        us = float(np.mean([s[0] for s in samples]))
+        # one launch per iteration: the solve's launch loop is this kernel
+        # (99.8% of its GPU time by rocprofv3), so its per-launch time is the
+        # loop's HIP-event time over the launches; a launch bracketed by its own
+        # events (the two-launch kernels' measure) reads ~1.3 us more
+        us = float(np.mean([s[2] if fused else s[0] for s in samples]))
         achieved = byts / (us * 1e-6) / 1e9
-        roofline = dict(bound="hbm", kernel="pcg_spmv_kernel (FP64 BSR SpMV + p.q partials)",
+        roofline = dict(bound="hbm", kernel=kname,
                         regime=("latency-bound: the 31 MB matrix and the vectors stay in the "
                                 "256 MB Infinity Cache at N=100k; the fraction is algorithmic "
                                 "bytes over time against the 8 TB/s HBM peak (counted HBM "
                                 "traffic in 'traffic')"),
                         us_per_launch=us, bytes_per_launch=byts, achieved=achieved,
                         peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                        traffic=measured_traffic(),
-                        how=f"HIP events around every {args.profile_every}th SpMV launch of "
-                            f"the {args.steps} timed FP64 solves")
-        log(f"spmv64 in solve: {us:.2f} us/launch, {achieved:.0f} GB/s algorithmic")
+                        traffic=measured_traffic(fused),
+                        how=(f"HIP events around the launch loop of the {args.steps} timed FP64 "
+                             f"solves, per launch (events around every {args.profile_every}th "
+                             f"launch: us_per_launch_sampled)" if fused else
+                             f"HIP events around every {args.profile_every}th launch of "
+                             f"the {args.steps} timed FP64 solves"))
+        if fused:
+            roofline["us_per_launch_sampled"] = float(np.mean([s[0] for s in samples]))
+        log(f"{'fused iteration' if fused else 'spmv64'} in solve: {us:.2f} us/launch, "
+            f"{achieved:.0f} GB/s algorithmic")
     g.set_poses(init)
     g.assemble()
     kernels = {k: g.time_kernel(k, 200) for k in ("spmv64", "update64", "spmv_mixed", "assembly")}
-    if samples:
+    if fused:
+        kernels["fused64_in_solve"] = float(np.mean([s[2] for s in samples]))
+    elif samples:
         kernels["spmv64_in_solve"] = float(np.mean([s[0] for s in samples]))
         kernels["update64_in_solve"] = float(np.mean([s[1] for s in samples]))
     g.close()
@@ -342,7 +381,7 @@ def gj_section(args, gslam, G, np):
             nx = np.linalg.norm(x_np)
             row = dict(n=n, numpy_solve_s=t_np)
             for name, kw in (("ir3", dict(refine_sweeps=3, refine_mode="ir", tol=0.0)),
-                             ("cg", dict(refine_sweeps=400, refine_mode="cg", tol=2e-12))):
+                             ("cg", dict(refine_sweeps=400, refine_mode="cg", tol=1e-12))):
                 g.solve("gj_mixed", **kw)   # warm-up (work space allocation)
                 st = g.solve("gj_mixed", **kw)
                 e = float(np.linalg.norm(g.dx() - x_np) / nx)
@@ -689,7 +728,7 @@ def profile_kernel(args, gslam, G):
         with gslam.Graph(gc["poses"], **G.graph_kwargs(gc, lam)) as g:
             g.assemble()
             for _ in range(3):
-                st = g.solve("pcg64", 1e-10, 400000, pcg_schedule=args.pcg_schedule)
+                st = g.solve("pcg64", 1e-10, args.pcg_iters, pcg_schedule=args.pcg_schedule)
                 log(f"pcg_solve {args.graph} N={args.n} ({args.pcg_schedule}): {st['ms']:.2f} ms, "
                     f"{st['iters']} its, {1000 * st['ms'] / max(st['iters'], 1):.2f} us per iteration")
         return
@@ -727,7 +766,7 @@ def main():
                          "or reverse Cuthill-McKee")
     ap.add_argument("--tol", type=float, default=3e-13, help="FP64 PCG tolerance")
     ap.add_argument("--mixed-tol", type=float, default=3e-13)
-    ap.add_argument("--handover", type=float, default=1e-4,
+    ap.add_argument("--handover", type=float, default=1e-3,
                     help="mixed: FP64 true relative residual at which FP64 PCG takes over the "
                          "correction (0: FP32 iterations all the way)")
     ap.add_argument("--park-n", type=int, default=7120, help="park section size (0 skips)")
@@ -761,6 +800,8 @@ def main():
                     help="--profile-kernel pcg_solve: grid city or the park graph (laps route)")
     ap.add_argument("--pcg-schedule", choices=["auto", "two_launch"], default="auto",
                     help="--profile-kernel pcg_solve: FP64 PCG launch schedule")
+    ap.add_argument("--pcg-iters", type=int, default=400000,
+                    help="--profile-kernel pcg_solve: iteration cap (a short run for counters)")
     ap.add_argument("--profile-kernel", choices=["spmv64", "update64", "spmv_mixed", "pcg_solve", "gj_trailing",
                                                  "assembly", "copy", "stream"],
                     help="only launch this kernel a few times and exit (rocprofv3 counter "

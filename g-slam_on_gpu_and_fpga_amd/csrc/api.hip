@@ -79,7 +79,7 @@ int DeviceGraph::fused_reserve()
     GSLAM_TRY(alloc(&q_alt, dim()));
     GSLAM_TRY(alloc(&zwp, 2 * 3 * dim()));
     double *f = nullptr;
-    GSLAM_TRY(alloc(&f, static_cast<int64_t>(2 * kFusedParts) * kFusedMaxTiles));
+    GSLAM_TRY(alloc(&f, static_cast<int64_t>(2 * kFusedParts) * kFusedMaxGroups));
     fpart = f;  // last: fpart != nullptr means every fused buffer exists
     return GSLAM_OK;
 }
@@ -354,6 +354,7 @@ int gslam_get_info(const gslam_graph *gh, gslam_graph_info *info)
     info->n_blocks = h.n_blocks;
     info->nnz = h.nnz;
     info->device_bytes = gh->g.bytes + gh->g.dense.bytes + gh->g.dense64.bytes;
+    info->n_tiles = gh->g.n_tiles;
     return GSLAM_OK;
 }
 

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cmath>
+#include <type_traits>
 
 namespace gslam {
 
@@ -520,69 +521,153 @@ __device__ inline double fused_p(double z, double w, double p_old, double alpha,
     return fma(beta, p_old, fma(-alpha, w, z));
 }
 
-__global__ void __launch_bounds__(kThreads)
+// base + byte offset, the offset a 32-bit value: the form the compiler turns
+// into a scalar base register plus one 32-bit offset register (instead of a
+// 64-bit address pair per load).
+template <typename T>
+__device__ inline T *at_byte(T *base, uint32_t off)
+{
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return reinterpret_cast<T *>(reinterpret_cast<B *>(base) + off);
+}
+
+// A 3x3 block V (row-major) times the three parts of its column's record
+// e = {z, w, p}: y = {V z, V w, V p}, formed as soon as the loads arrive,
+// before alpha and beta are known (nine values held instead of eighteen).
+__device__ inline void fused_products(const double (&v)[9], const double (&e)[9], double (&y)[9])
+{
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            y[3 * s + r] = v[3 * r] * e[3 * s] + v[3 * r + 1] * e[3 * s + 1] + v[3 * r + 2] * e[3 * s + 2];
+}
+
+// Row r of V p_L = V z - alpha V w + beta V p.
+__device__ inline double fused_combine(const double (&y)[9], int r, double alpha, double beta)
+{
+    return fma(beta, y[6 + r], fma(-alpha, y[3 + r], y[r]));
+}
+
+// kTpw consecutive tiles per workgroup of kTpw x kThreads threads: part
+// w = tid / kThreads of the workgroup takes tile kTpw g + w of tile group g,
+// with the tile kernel's lanes and LDS layout.  kTpw = 4 keeps the partials
+// of the N = 100k grid city (1,697 tiles) at one per group (425 <= 512);
+// held to 64 VGPRs, two such groups share a CU, so all of them run in one
+// round.  Groups map onto the XCDs contiguously (neighbouring rows in one
+// L2), and the partials are indexed by group.
+template <int kTpw>
+__global__ void __launch_bounds__(kTpw * kThreads)
+__attribute__((amdgpu_waves_per_eu(kTpw == 1 ? 1 : 8, 8)))
 pcg_fused_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict__ bcol,
                  const double *__restrict__ bval, const int4 *__restrict__ tile_info,
                  const int32_t *__restrict__ tcol, const double *__restrict__ tval,
-                 const double *__restrict__ dinv, int32_t n, const double *__restrict__ r_in,
-                 const double *__restrict__ q_in, const double *__restrict__ rec_in,
-                 double *__restrict__ r_out, double *__restrict__ q_out,
-                 double *__restrict__ rec_out, double *__restrict__ x, int32_t L, double tol_rel,
-                 const double *__restrict__ part_in, int32_t np_in, double *__restrict__ part_out,
-                 PcgStatus *status)
+                 int32_t n_tiles, const double *__restrict__ dinv, int32_t n,
+                 const double *__restrict__ r_in, const double *__restrict__ q_in,
+                 const double *__restrict__ rec_in, double *__restrict__ r_out,
+                 double *__restrict__ q_out, double *__restrict__ rec_out, double *__restrict__ x,
+                 int32_t L, double tol_rel, const double *__restrict__ part_in, int32_t np_in,
+                 double *__restrict__ part_out, PcgStatus *status)
 {
+    constexpr int NT = kTpw * kThreads;
+    constexpr int kLoads = (kFusedMaxGroups + NT - 1) / NT;  // partials per thread and quantity
     static_assert(kTileBlocks == kThreads, "one lane per padded block slot");
-    static_assert(kFusedMaxTiles <= 2 * kThreads, "two partials per thread and quantity");
-    __shared__ double ys[3 * kTileBlocks];
-    __shared__ double xs[2][kThreads];  // the pose's r_L, then q_L entries
-    __shared__ double red[kFusedParts][kThreads / kWave];
+    __shared__ double ys[kTpw][3 * kTileBlocks];
+    __shared__ double xs[2][NT];  // the pose's r_L, then q_L entries
+    __shared__ double red[kFusedParts][NT / kWave];
+    __shared__ int32_t more[kTpw];  // chunks of a block row longer than a tile, per part
     const int tid = static_cast<int>(threadIdx.x);
-    const int32_t t = static_cast<int32_t>(blockIdx.x);
+    const int w = tid / kThreads, lt = tid % kThreads;
+    const int32_t grp = xcd_contiguous(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+    const int32_t tw = kTpw * grp + w;
+    const bool live = tw < n_tiles;  // the last group may hold fewer than kTpw tiles
+    const int32_t t = live ? tw : n_tiles - 1;
     const int64_t dim = 3 * static_cast<int64_t>(n);
     const int32_t done_in = status->done;
     const double tol_in = status->tol;
     // the partials of launch L-1 first (launch 0: ||b||^2 of the setup
-    // kernel, np_in <= 2 x kThreads), as unconditional loads from clamped
+    // kernel, np_in <= kFusedMaxGroups), as unconditional loads from clamped
     // indices, summed below in the same order in every workgroup
-    const int ia = min(tid, np_in - 1), ib = min(tid + kThreads, np_in - 1);
-    const bool in_a = tid < np_in, in_b = tid + kThreads < np_in;
-    double pa[kFusedParts], pb[kFusedParts];
+    double pa[kLoads][kFusedParts];
 #pragma unroll
-    for (int k = 0; k < kFusedParts; ++k) {
-        const int64_t off = L == 0 ? 0 : static_cast<int64_t>(k) * kFusedMaxTiles;
-        pa[k] = part_in[off + ia];
-        pb[k] = part_in[off + ib];
+    for (int u = 0; u < kLoads; ++u) {
+        const uint32_t iu = 8u * static_cast<uint32_t>(min(tid + u * NT, np_in - 1));
+#pragma unroll
+        for (int k = 0; k < kFusedParts; ++k)
+            pa[u][k] = *at_byte(part_in + (L == 0 ? 0 : k * kFusedMaxGroups), iu);
     }
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
     // this lane's block (tile-padded slot: zeros and a valid column in padding)
-    // and the gathered record of its column
+    // and the gathered record of its column, multiplied out on arrival
     const int4 ti = tile_info[t];
-    const int64_t slot = static_cast<int64_t>(t) * kTileBlocks + tid;
-    const double *ge = rec_in + 9 * static_cast<int64_t>(tcol[slot]);
-    const double *tv = tval + 9 * static_cast<int64_t>(t) * kTileBlocks + tid;
-    double v[9], gz[9];
+    double y[9];
+    {
+        // (32-bit byte offsets: the tile copy holds at most kFusedMaxTiles x
+        // 18 KB, the records at most 72 B x ~120k poses)
+        const uint32_t slot = static_cast<uint32_t>(t) * kTileBlocks + lt;
+        const uint32_t col = static_cast<uint32_t>(*at_byte(tcol, 4u * slot));
+        // the partials return before the column index (loads complete in
+        // issue order): their wave totals go to LDS meanwhile, which frees
+        // their registers; the block's values are loaded with the gathered
+        // record, whose address waits for the column index anyway
+        double pv[kFusedParts];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] = tv[k * kTileBlocks];
+        for (int k = 0; k < kFusedParts; ++k) {
+            pv[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) gz[k] = ge[k];
-    // this thread's output entry (row a of pose orow, clamped into the tile)
-    const int a = tid % 3;
-    const bool has_blk = ti.z + tid < ti.w, has_out = tid < 3 * (ti.y - ti.x);
-    const int32_t orow = min(ti.x + tid / 3, ti.y - 1);
+            for (int u = 0; u < kLoads; ++u) pv[k] += tid + u * NT < np_in ? pa[u][k] : 0.0;
+        }
+        wave_totals<kFusedParts, NT>(pv, red);
+        asm volatile("" ::: "memory");
+        const uint32_t tv = 8u * (static_cast<uint32_t>(t) * (9u * kTileBlocks) + lt);
+        double v[9], e[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = *at_byte(tval + k * kTileBlocks, tv);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) e[k] = *at_byte(rec_in + k, 72u * col);
+        fused_products(v, e, y);
+    }
+    if constexpr (kTpw > 1) {
+        // the products are formed here (not sunk to their use after the
+        // reduction), so that their 18 inputs are dead before the own-row
+        // loads
+#pragma unroll
+        for (int k = 0; k < 9; ++k) asm volatile("" ::"v"(y[k]));
+    }
+    // this thread's output entry (row a of pose orow, clamped into the tile);
+    // its row range first, its values (used after the row sums) early in the
+    // one-tile form and after the block products in the wide one, which
+    // keeps the products' 18 inputs and the partials out of the registers
+    // meanwhile (64 VGPRs, no spills)
+    const int a = lt % 3;
+    const bool has_blk = live && ti.z + lt < ti.w, has_out = live && lt < 3 * (ti.y - ti.x);
+    const int32_t orow = min(ti.x + lt / 3, ti.y - 1);
     const int32_t ob0 = brow_ptr[orow], ob1 = brow_ptr[orow + 1];
     const int64_t o = 3 * static_cast<int64_t>(orow) + a;
-    const double *oe = rec_in + 9 * static_cast<int64_t>(orow);
-    const double oz = oe[a], ow = oe[3 + a], op = oe[6 + a];
-    const double orr = r_in[o], oq = q_in[o], ox = x[o];
-    const double od0 = dinv[o], od1 = dinv[dim + o], od2 = dinv[2 * dim + o];  // row a of D^-1
+    double oz, ow, op, orr, oq, ox, od0, od1, od2;
+    // (32-bit byte offsets into the vectors: 3N <= 3 x ~120k entries)
+    const uint32_t ob = 8u * static_cast<uint32_t>(o);
+    const uint32_t oeb = 8u * (9u * static_cast<uint32_t>(orow) + static_cast<uint32_t>(a));
+    auto load_own = [&]() {
+        oz = *at_byte(rec_in, oeb);
+        ow = *at_byte(rec_in + 3, oeb);
+        op = *at_byte(rec_in + 6, oeb);
+        orr = *at_byte(r_in, ob);
+        oq = *at_byte(q_in, ob);
+        ox = *at_byte(static_cast<const double *>(x), ob);
+        od0 = *at_byte(dinv, ob);  // row a of D^-1
+        od1 = *at_byte(dinv + dim, ob);
+        od2 = *at_byte(dinv + 2 * dim, ob);
+    };
+    if constexpr (kTpw == 1) load_own();
+    if (lt == 0) more[w] = live ? max(0, (ti.w - ti.z - 1) / kTileBlocks) : 0;
     double pv[kFusedParts];
-#pragma unroll
-    for (int k = 0; k < kFusedParts; ++k) pv[k] = (0.0 + (in_a ? pa[k] : 0.0)) + (in_b ? pb[k] : 0.0);
-    block_allsum<kFusedParts, kThreads>(pv, red);
+    __syncthreads();
+    sum_wave_totals<kFusedParts, NT>(pv, red);
     if (done_in) return;
     const double pq = pv[0], rz = pv[1], qz = pv[2], qw = pv[3], rr = L == 0 ? pv[0] : pv[4];
     double alpha = 0.0, beta = 0.0;
-    const bool lead = t == 0 && tid == 0;
+    const bool lead = grp == 0 && tid == 0;
     if (L == 0) {
         if (lead) {
             status->bnorm = sqrt(rr);
@@ -618,49 +703,65 @@ pcg_fused_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict
         alpha = rz / pq;
         beta = fma(alpha * alpha, qw, fma(-2.0 * alpha, qz, rz)) / rz;
     }
-    // p_L of the gathered column, its products into LDS
+    // V p_L of this lane's block into LDS
+    double *yl = ys[w];
     if (has_blk) {
-        double pj[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pj[c] = fused_p(gz[c], gz[3 + c], gz[6 + c], alpha, beta);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-            ys[r * kTileBlocks + tid] = v[3 * r] * pj[0] + v[3 * r + 1] * pj[1] + v[3 * r + 2] * pj[2];
+        for (int r = 0; r < 3; ++r) yl[r * kTileBlocks + lt] = fused_combine(y, r, alpha, beta);
     }
-    // own entry: r_L (to LDS for z_L = D^-1 r_L), p_L
-    const double r_a = fma(-alpha, oq, orr);
-    const double p_a = fused_p(oz, ow, op, alpha, beta);
-    xs[0][tid] = r_a;
+    if constexpr (kTpw > 1) load_own();
     __syncthreads();
     double acc = 0.0;
     if (has_out) {
         const int32_t hi = min(ob1, ti.z + kTileBlocks);
-        for (int32_t c = max(ob0, ti.z); c < hi; ++c) acc += ys[a * kTileBlocks + (c - ti.z)];
+        for (int32_t c = max(ob0, ti.z); c < hi; ++c) acc += yl[a * kTileBlocks + (c - ti.z)];
     }
     // a single block row longer than one tile: further chunks from the BSR
-    // arrays (the condition is the same in the whole workgroup)
-    if (ti.w > ti.z + kTileBlocks) __syncthreads();
-    for (int32_t cb = ti.z + kTileBlocks; cb < ti.w; cb += kTileBlocks) {
-        const int32_t bb = cb + tid;
-        if (bb < ti.w) {
+    // arrays, as many rounds as the longest of the workgroup's tiles needs
+    // (the same count, hence the same barriers, in every thread)
+    int32_t n_more = 0;
+#pragma unroll
+    for (int i = 0; i < kTpw; ++i) n_more = max(n_more, more[i]);
+    if (n_more > 0) __syncthreads();
+    for (int32_t m = 1; m <= n_more; ++m) {
+        const int32_t cb = ti.z + m * kTileBlocks;
+        const int32_t bb = cb + lt;
+        if (live && bb < ti.w) {
+            // (rows this long are rare -- a pose with more than kTileBlocks
+            // edge ends --: three values per round trip, so that this path
+            // needs few registers beside the own-row values; p_L as fused_p
+            // rounds it)
             const double *ee = rec_in + 9 * static_cast<int64_t>(bcol[bb]);
             double pj[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) pj[c] = fused_p(ee[c], ee[3 + c], ee[6 + c], alpha, beta);
+            for (int c = 0; c < 3; ++c) pj[c] = ee[c];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pj[c] = fma(-alpha, ee[3 + c], pj[c]);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pj[c] = fma(beta, ee[6 + c], pj[c]);
             const double *bv = bval + 9 * static_cast<int64_t>(bb);
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-                ys[r * kTileBlocks + tid] = bv[3 * r] * pj[0] + bv[3 * r + 1] * pj[1] + bv[3 * r + 2] * pj[2];
+            for (int r = 0; r < 3; ++r) {
+                asm volatile("" ::: "memory");
+                yl[r * kTileBlocks + lt] = bv[3 * r] * pj[0] + bv[3 * r + 1] * pj[1] + bv[3 * r + 2] * pj[2];
+            }
         }
         __syncthreads();
         if (has_out) {
             const int32_t lo = max(ob0, cb), hi = min(ob1, cb + kTileBlocks);
-            for (int32_t c = lo; c < hi; ++c) acc += ys[a * kTileBlocks + (c - cb)];
+            for (int32_t c = lo; c < hi; ++c) acc += yl[a * kTileBlocks + (c - cb)];
         }
         __syncthreads();
     }
-    // q_L to LDS for w_L = D^-1 q_L; both D^-1 products from the pose's three
-    // entries (tid - a is the pose's first thread)
+    // own entry: r_L and p_L; r_L and q_L to LDS for z_L = D^-1 r_L and
+    // w_L = D^-1 q_L
+    const double r_a = fma(-alpha, oq, orr);
+    const double p_a = fused_p(oz, ow, op, alpha, beta);
+    xs[0][tid] = r_a;
+    // (both D^-1 products from the pose's three entries: tid - a is the
+    // pose's first thread, in the same tile's part)
     xs[1][tid] = acc;
     __syncthreads();
     double part[kFusedParts] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -668,20 +769,19 @@ pcg_fused_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict
         const int base = tid - a;
         const double z_a = fma(od2, xs[0][base + 2], fma(od1, xs[0][base + 1], od0 * xs[0][base]));
         const double w_a = fma(od2, xs[1][base + 2], fma(od1, xs[1][base + 1], od0 * xs[1][base]));
-        r_out[o] = r_a;
-        q_out[o] = acc;
-        double *re = rec_out + 9 * static_cast<int64_t>(orow);
-        re[a] = z_a;
-        re[3 + a] = w_a;
-        re[6 + a] = p_a;
-        if (L > 0) x[o] = fma(alpha, op, ox);
+        *at_byte(r_out, ob) = r_a;
+        *at_byte(q_out, ob) = acc;
+        *at_byte(rec_out, oeb) = z_a;
+        *at_byte(rec_out + 3, oeb) = w_a;
+        *at_byte(rec_out + 6, oeb) = p_a;
+        if (L > 0) *at_byte(x, ob) = fma(alpha, op, ox);
         part[0] = p_a * acc;
         part[1] = r_a * z_a;
         part[2] = acc * z_a;
         part[3] = acc * w_a;
         part[4] = r_a * r_a;
     }
-    write_partials<kFusedParts>(part, part_out, kFusedMaxTiles, t);
+    write_partials<kFusedParts, NT>(part, part_out, kFusedMaxGroups, grp);
     if (lead) status->iters = L;
 }
 
@@ -693,7 +793,7 @@ pcg_fused_finalize_kernel(const double *__restrict__ part_last, int32_t np, int3
 {
     if (status->done || launches < 2) return;  // launch 0's partials hold r_0 = b
     double cur[1];
-    sum_partials<1>(part_last + 4 * kFusedMaxTiles, np, kFusedMaxTiles, cur);
+    sum_partials<1>(part_last + 4 * kFusedMaxGroups, np, kFusedMaxGroups, cur);
     if (threadIdx.x == 0) {
         const int32_t it = launches - 1;
         status->iters = it;
@@ -899,19 +999,24 @@ int pose_update(DeviceGraph &g, double *diff_out)
 }
 
 // One kernel per iteration (pcg_fused_kernel) for graphs of at most
-// kFusedMaxTiles tiles.
+// kFusedMaxTiles tiles: one tile per workgroup up to kFusedMaxGroups tiles,
+// kFusedWideTiles beyond.
 static int pcg_solve_fused(DeviceGraph &g, const gslam_solve_opts &o, const double *b,
                            double *x_out, gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
     const unsigned int ge = grid_for(n, 1);
-    const unsigned int gt = static_cast<unsigned int>(g.n_tiles);
+    const bool wide = g.n_tiles > kFusedMaxGroups;
+    const int tpw = wide ? kFusedWideTiles : 1;
+    const unsigned int gt = static_cast<unsigned int>((g.n_tiles + tpw - 1) / tpw);
+    const dim3 blk(tpw * kThreads);
+    auto kern = wide ? pcg_fused_kernel<kFusedWideTiles> : pcg_fused_kernel<1>;
     PcgStatus *st = g.status;
     GSLAM_TRY(g.tval_reserve());  // first solve: outside the timed region
     GSLAM_TRY(g.fused_reserve());
     double *R[2] = {g.r, g.r_alt}, *Q[2] = {g.q, g.q_alt};
     double *E[2] = {g.zwp, g.zwp + 3 * g.dim()};
-    double *FP[2] = {g.fpart, g.fpart + kFusedParts * kFusedMaxTiles};
+    double *FP[2] = {g.fpart, g.fpart + kFusedParts * kFusedMaxGroups};
     GSLAM_HIP(hipEventRecord(g.ev0, g.stream));
     GSLAM_TRY(launch_tile_pack(g));
     // launch 0 reads r_(-1) = b, q_(-1) = 0, the record {D^-1 b, 0, 0} (parity
@@ -937,14 +1042,14 @@ static int pcg_solve_fused(DeviceGraph &g, const gslam_solve_opts &o, const doub
             auto launch = [&](auto... a) {
                 if (samp) {
                     hipEvent_t *ev = &g.prof_ev[4 * n_samp++];
-                    hipExtLaunchKernelGGL(pcg_fused_kernel, dim3(gt), dim3(kThreads), 0, g.stream, ev[0],
-                                          ev[1], 0, a...);
+                    hipExtLaunchKernelGGL(kern, dim3(gt), blk, 0, g.stream, ev[0], ev[1], 0, a...);
                 } else {
-                    hipLaunchKernelGGL(pcg_fused_kernel, dim3(gt), dim3(kThreads), 0, g.stream, a...);
+                    hipLaunchKernelGGL(kern, dim3(gt), blk, 0, g.stream, a...);
                 }
             };
-            launch(g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.dinv, n, R[prev], Q[prev],
-                   E[prev], R[cur], Q[cur], E[cur], x_out, k, o.tol, pin, npin, FP[cur], st);
+            launch(g.brow_ptr, g.bcol, g.bval, g.tile_info, g.tcol, g.tval, g.n_tiles, g.dinv, n,
+                   R[prev], Q[prev], E[prev], R[cur], Q[cur], E[cur], x_out, k, o.tol, pin, npin,
+                   FP[cur], st);
         }
         hipLaunchKernelGGL(pcg_fused_finalize_kernel, dim3(1), dim3(kThreads), 0, g.stream,
                            FP[(k - 1) & 1], static_cast<int32_t>(gt), k, o.max_iter, st);

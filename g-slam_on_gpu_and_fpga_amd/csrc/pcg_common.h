@@ -29,10 +29,14 @@ constexpr int kPartTotal = kPartOffRz + 8;
 constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 
 // The FP64 PCG of graphs with at most kFusedMaxTiles SpMV tiles (N up to
-// ~30k) runs one kernel per iteration: every workgroup re-reads all
-// kFusedParts partials of the previous launch (at most 10 loads per
-// thread), so the update of r, z, p and x needs no launch of its own.
-constexpr int kFusedMaxTiles = 512;
+// ~120k) runs one kernel per iteration: every workgroup re-reads all
+// kFusedParts partials of the previous launch (one per workgroup, at most
+// kFusedMaxGroups; 10 loads per thread at most), so the update of r, z, p
+// and x needs no launch of its own.  A workgroup takes one tile up to
+// kFusedMaxGroups tiles, kFusedWideTiles consecutive tiles beyond.
+constexpr int kFusedMaxGroups = 512;
+constexpr int kFusedWideTiles = 4;
+constexpr int kFusedMaxTiles = kFusedWideTiles * kFusedMaxGroups;
 constexpr int kFusedParts = 5;  // p.q, r.z, q.z, q.D^-1 q, r.r
 
 // Arrays that a PCG kernel reads once per iteration (the matrix values and

@@ -56,7 +56,8 @@ ASSEMBLY = {"auto": 0, "pose_walk": 1, "staged": 2}
 class _Info(C.Structure):
     _fields_ = [("n_poses", C.c_int32), ("n_edges", C.c_int64), ("n_obs_edges", C.c_int64),
                 ("n_motion_edges", C.c_int64), ("n_between_edges", C.c_int64),
-                ("n_blocks", C.c_int64), ("nnz", C.c_int64), ("device_bytes", C.c_int64)]
+                ("n_blocks", C.c_int64), ("nnz", C.c_int64), ("device_bytes", C.c_int64),
+                ("n_tiles", C.c_int32)]
 
 
 class _Opts(C.Structure):

@@ -108,6 +108,9 @@ typedef struct {
     int64_t n_blocks;           /* 3x3 blocks of Omega (BSR)                 */
     int64_t nnz;                /* scalar nonzeros of the reference CSR      */
     int64_t device_bytes;       /* device memory held by the handle          */
+    int32_t n_tiles;            /* SpMV tiles of whole block rows (<= 256
+                                   blocks each); the FP64 PCG runs one launch
+                                   per iteration up to 2048 of them          */
 } gslam_graph_info;
 
 #define GSLAM_SOLVER_PCG_FP64   0  /* block-Jacobi PCG, FP64 throughout      */

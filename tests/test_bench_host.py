@@ -16,6 +16,10 @@ def test_byte_counts():
     assert bench.spmv_bytes(100000, 430854) == 47544908
     # per pose r, q and D^-1 read, r and z written
     assert bench.update_bytes(100000) == 16800000
+    # one-launch iteration: the SpMV's blocks and row pointers; per pose the
+    # record {z, w, p}, r/q/x and D^-1 read, r/q/x and the record written
+    assert bench.fused_bytes(1, 1) == 76 + 8 + 5 * 72
+    assert bench.fused_bytes(100000, 430854) == 69144908
     # per incidence entry its edge, other pose and 8 FP32 factors; per pose
     # z and p_old read and p_new and q written (4 x 3 floats)
     assert bench.mixed_spmv_bytes(1, 2) == 2 * 40 + 48

@@ -140,6 +140,91 @@ def test_pcg64_one_launch_schedule_against_oracle(oracle):
         assert abs(its["auto"] - its["two_launch"]) <= 0.02 * its["two_launch"], its
 
 
+def _wide_graph(gc):
+    """A graph of more than 512 SpMV tiles (one launch per iteration with four
+    consecutive tiles per workgroup) whose tile count is not a multiple of
+    four (the last workgroup runs with idle parts)."""
+    g = gslam.Graph(gc["poses"], **G.graph_kwargs(gc, 0.1))
+    nt = g.info["n_tiles"]
+    assert 512 < nt <= 2048, nt
+    return g, nt
+
+
+def test_pcg64_four_tiles_per_workgroup_against_oracle(oracle):
+    """Grid city N = 40,000 (about 680 SpMV tiles).  Fifty iterations of both
+    schedules against fifty of the CPU oracle on the same system; both to tol
+    1e-12 within 1e-8 of the direct solve and within 5% of each other's
+    iteration count (31,021 and 31,840 measured); iteration caps and the
+    p.q == 0 break of a zero right-hand side."""
+    for n in (40000, 40013, 40029, 40047, 40061):
+        gc = G.grid_city(n, seed=10)
+        g, nt = _wide_graph(gc)
+        if nt % 4:
+            break
+        g.close()
+    assert nt % 4, nt
+    with g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
+        assert it_o == 50
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-30, 50, pcg_schedule=schedule)
+            assert st["iters"] == 50 and not st["converged"]
+            assert rel(g.dx(), x_o) <= 1e-9, schedule
+        xs = direct(rp, ci, va, xi)
+        its = {}
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-12, 400000, pcg_schedule=schedule)
+            assert st["converged"] and st["true_rel_res"] < 2e-12, schedule
+            assert rel(g.dx(), xs) <= 1e-8, schedule
+            its[schedule] = st["iters"]
+        assert abs(its["auto"] - its["two_launch"]) <= 0.05 * its["two_launch"], its
+        for cap in (1, 2, 7):
+            st = g.solve("pcg64", 1e-14, cap)
+            assert st["iters"] == cap and not st["converged"], cap
+        x, st = g.solve_rhs(np.zeros(len(xi)), "pcg64", 1e-6, 100)
+        assert not x.any() and st["flags"] & 1 and st["iters"] == 0
+
+
+def test_pcg64_four_tiles_per_workgroup_long_block_row(oracle):
+    """Grid city N = 40,000 plus 700 loop closures from pose 20,000 to poses
+    drawn across the walk: its block row (~710 blocks) takes three rounds of
+    256, in a workgroup whose other tiles take one, so every part of that
+    workgroup runs the extra rounds' barriers.  Fifty iterations of both
+    schedules against fifty of the CPU oracle; then both converged (tol
+    1e-12) against the extended-precision refined direct solve.  The hub's
+    lever arms of up to a few hundred metres leave this Omega ill-conditioned:
+    the CPU oracle's own converged PCG lands 1.4e-7 from the refined solve
+    (and the plain LU solve with one refinement step 1.6e-7), hence the bound
+    of 5e-7."""
+    base = G.grid_city(40000, seed=11)
+    rng = np.random.default_rng(12)
+    hub = 20000
+    others = np.sort(rng.choice(np.setdiff1d(np.arange(40000), np.arange(hub - 30, hub + 31)),
+                                700, replace=False))
+    lc = np.flatnonzero(np.arange(len(base["i"])) >= base["n_odometry"])
+    I = np.concatenate([base["i"][lc], np.full(700, hub)])
+    J = np.concatenate([base["j"][lc], others])
+    gc = G._finish(base["truth"], I, J, rng, 0.05, 0.01)
+    g, nt = _wide_graph(gc)
+    with g:
+        g.assemble()
+        rp, ci, va, xi = g.system_csr()
+        assert rp[3 * hub + 1] - rp[3 * hub] >= 3 * 700
+        x_o, it_o = oracle.pcg(rp, ci, va, xi, tol=1e-30, max_iter=50)
+        assert it_o == 50
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-30, 50, pcg_schedule=schedule)
+            assert st["iters"] == 50 and not st["converged"]
+            assert rel(g.dx(), x_o) <= 1e-9, schedule
+        xr = refined_direct(rp, ci, va, xi)
+        for schedule in SCHEDULES:
+            st = g.solve("pcg64", 1e-12, 400000, pcg_schedule=schedule)
+            assert st["converged"] and st["true_rel_res"] < 2e-12, schedule
+            assert rel(g.dx(), xr) <= 5e-7, (schedule, rel(g.dx(), xr))
+
+
 def test_nontemporal_streams_against_oracle(oracle):
     """Grid city N = 2,000,000: the tile copy (~700 MB), the six FP64 solver
     vectors (288 MB) and the FP32 operator's slots all exceed their
