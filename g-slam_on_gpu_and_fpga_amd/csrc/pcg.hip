@@ -20,8 +20,9 @@
 //                  later.
 // Kernel boundaries order the partials; a grid-wide reduction through one
 // atomic counter costs ~15-35 us per iteration on MI355X at 1-2k
-// workgroups, the redundant re-read of <= 512 partials ~1 us.  Only
-// workgroup 0 writes the status record the host polls.
+// workgroups, the redundant re-read of <= 512 partials ~1 us (hence at most
+// kFusedMaxGroups workgroups in the one-launch form).  Only workgroup 0 (or
+// tile group 0) writes the status record the host polls.
 #include <hip/hip_ext.h>
 
 #include "device_common.h"

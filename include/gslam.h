@@ -146,9 +146,9 @@ typedef struct {
 } gslam_solve_opts;
 
 /* FP64 PCG launch schedules (same block-Jacobi PCG, same stopping rule):
- *   AUTO: one kernel per iteration on graphs of at most 512 SpMV tiles (about
- *         30,000 poses), where kernel launches bound the iteration; beta then
- *         comes from the one-step identity r.z' = r.z - 2 alpha q.z +
+ *   AUTO: one kernel per iteration on graphs of at most 2048 SpMV tiles
+ *         (about 120,000 poses; gslam_graph_info.n_tiles); beta then comes
+ *         from the one-step identity r.z' = r.z - 2 alpha q.z +
  *         alpha^2 q.D^-1 q, so the iterates differ from the two-launch ones
  *         in the last bits; two kernels per iteration on larger graphs.
  *   TWO_LAUNCH: two kernels per iteration at every size. */
