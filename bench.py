@@ -14,11 +14,12 @@ refined on residuals in x87 extended precision: ~1e-11 relative; the FP64
 LU solve alone is 1.3e-7 away), with the FP64 PCG solve at tolerance 1e-14
 reported beside it.  The FP64 solver runs at tolerance 3e-13 (3.3e-9 rel dx
 err on this graph; 1e-12 gives ~1.3e-8); the mixed solver runs FP32
-iterations until the FP64 true residual is below 1e-3 and then FP64 PCG on
-the correction to the same 3e-13 (7.9e-9; from 1e-4 the correction ends at
-1.1-1.4e-8 -- at kappa ~ 1e13 the attainable error is ~1e-8 within a factor
-of three, set by the rounding path -- and FP32 iterations alone stall at
-~1e-6 error, kappa far beyond 1/eps32).
+iterations until the FP64 true residual is below 1e-2 (as on the park
+graph) and then FP64 PCG on the correction to the same 3e-13 (7.8e-9;
+hand-overs at 3e-3 and 1e-3 also meet 1e-8, 3e-2 and 1e-4 end at 1.2-1.5e-8
+-- at kappa ~ 1e13 the attainable error is ~1e-8 within a factor of three,
+set by the rounding path -- and FP32 iterations alone stall at ~1e-6
+error, kappa far beyond 1/eps32).
 
 Also reported:
 * roofline: the dominant FP64 PCG kernel's mean per-launch duration inside
@@ -766,7 +767,7 @@ def main():
                          "or reverse Cuthill-McKee")
     ap.add_argument("--tol", type=float, default=3e-13, help="FP64 PCG tolerance")
     ap.add_argument("--mixed-tol", type=float, default=3e-13)
-    ap.add_argument("--handover", type=float, default=1e-3,
+    ap.add_argument("--handover", type=float, default=1e-2,
                     help="mixed: FP64 true relative residual at which FP64 PCG takes over the "
                          "correction (0: FP32 iterations all the way)")
     ap.add_argument("--park-n", type=int, default=7120, help="park section size (0 skips)")

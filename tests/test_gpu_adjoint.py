@@ -114,3 +114,30 @@ def test_torch_function_and_adam():
         _, lam_out, hist = AD.tune(g, gc["poses"], gc["truth"], steps=8, lr=0.05, solve_kw=CONV)
     assert hist[-1] < hist[0]
     assert lam_out > 0
+
+
+def test_backward_after_a_second_forward_on_the_same_graph():
+    """Two forwards on one graph (other weights, other lambda), then backward
+    through the first: the gradient is the first forward's, because backward
+    restores that forward's weights, lambda and converged poses on the graph
+    before the adjoint solve."""
+    gc = G.grid_city(300, seed=9)
+    rng = np.random.default_rng(10)
+    with _graph(gc) as g:
+        w1 = torch.zeros(len(gc["i"]), dtype=torch.float32, requires_grad=True)
+        lam1 = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+        p1 = AD.ConvergedPoses.apply(w1, lam1, g, gc["poses"], CONV)
+        w2 = torch.tensor(rng.normal(size=len(gc["i"])) * 0.3, dtype=torch.float32, requires_grad=True)
+        lam2 = torch.tensor(0.25, dtype=torch.float64, requires_grad=True)
+        p2 = AD.ConvergedPoses.apply(w2, lam2, g, gc["poses"], CONV)
+        assert not torch.allclose(p1, p2)
+        AD.drift_and_truth_loss(p1, gc["truth"]).backward()
+        g1w, g1l = w1.grad.clone(), lam1.grad.clone()
+    with _graph(gc) as h:
+        w = torch.zeros(len(gc["i"]), dtype=torch.float32, requires_grad=True)
+        lam = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+        p = AD.ConvergedPoses.apply(w, lam, h, gc["poses"], CONV)
+        assert torch.allclose(p, p1, rtol=0, atol=1e-9)
+        AD.drift_and_truth_loss(p, gc["truth"]).backward()
+    assert np.allclose(g1w.numpy(), w.grad.numpy(), rtol=1e-5, atol=1e-9)
+    assert abs(float(g1l) - float(lam.grad)) <= 1e-9 + 1e-5 * abs(float(lam.grad))

@@ -104,25 +104,26 @@ def test_c4a_mixed_grid100k(grid100k):
     """FP32 iterations with FP64 true-residual refreshes stall at ~1e-6 here
     (the block-Jacobi preconditioned operator has kappa ~ 1e13, far beyond
     1/eps32); with the FP64 correction (mixed_handover) the mixed solver meets
-    the FP64 solver's accuracy.  The handover at a true residual of 1e-3 is
-    the workload's: from 1e-4 the correction ends at 1.1-1.4e-8 (tol 1e-13 to
-    3e-13), from 1e-3 at 7.9e-9 -- at this conditioning the attainable error
-    is ~1e-8 within a factor of three, set by the rounding path."""
+    the FP64 solver's accuracy.  The handover at a true residual of 1e-2 is
+    the workload's: from 1e-2, 3e-3 and 1e-3 the correction ends at 7.8e-9,
+    7.9e-9 and 7.9e-9, from 3e-2 and 1e-4 at 1.5e-8 and 1.2e-8 -- at this
+    conditioning the attainable error is ~1e-8 within a factor of three, set
+    by the rounding path."""
     g, x_ref = grid100k
     st = g.solve("mixed", 1e-12, 1500000)
     e0 = rel(g.dx(), x_ref)
     assert st["converged"] and st["true_rel_res"] < 1e-12 and st["iters_fp64"] == 0
-    st = g.solve("mixed", 3e-13, 1500000, mixed_handover=1e-3)
+    st = g.solve("mixed", 3e-13, 1500000, mixed_handover=1e-2)
     dx = g.dx()
     e = rel(dx, x_ref)
     em, er = _maxerr(dx, x_ref)
     print(f"C4a mixed N=100k: FP32 only (tol 1e-12) rel dx err {e0:.3e}; with the FP64 "
-          f"correction below 1e-3 (tol 3e-13): {st['iters']} its ({st['iters_fp64']} FP64), "
+          f"correction below 1e-2 (tol 3e-13): {st['iters']} its ({st['iters_fp64']} FP64), "
           f"rel dx err {e:.3e}, max pose err {em:.2e} m {er:.2e} rad, true rel res "
           f"{st['true_rel_res']:.2e}")
     assert 1e-7 <= e0 <= 3e-6, e0                     # measured 3.6e-7 and 1.2e-6: the FP32 floor
     assert st["converged"] and 0 < st["iters_fp64"] < st["iters"]
-    assert e <= 1e-8, e                               # measured 7.9e-9
+    assert e <= 1e-8, e                               # measured 7.75e-9
 
 
 @pytest.mark.parametrize("solver,tol,kw", [("pcg64", 1e-12, {}), ("mixed", 1e-12, {}),
