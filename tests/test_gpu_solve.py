@@ -185,6 +185,19 @@ def test_pcg64_four_tiles_per_workgroup_against_oracle(oracle):
             assert st["iters"] == cap and not st["converged"], cap
         x, st = g.solve_rhs(np.zeros(len(xi)), "pcg64", 1e-6, 100)
         assert not x.any() and st["flags"] & 1 and st["iters"] == 0
+        # a NaN in the right-hand side, then an Inf in Omega, stop the
+        # four-tile iteration with GSLAM_ERR_NONFINITE
+        bad = xi.copy()
+        bad[3 * (len(xi) // 6) + 1] = np.nan
+        with pytest.raises(gslam.GSlamError) as ei:
+            g.solve_rhs(bad, "pcg64", 1e-10, 100000)
+        assert ei.value.code == gslam.ERR_NONFINITE
+        va_bad = va.copy()
+        va_bad[rp[len(xi) - 5]] = np.inf
+        g.set_system_csr(va_bad, xi)
+        with pytest.raises(gslam.GSlamError) as ei:
+            g.solve("pcg64", 1e-10, 100000)
+        assert ei.value.code == gslam.ERR_NONFINITE
 
 
 def test_pcg64_four_tiles_per_workgroup_long_block_row(oracle):
