@@ -552,11 +552,13 @@ __device__ inline double fused_combine(const double (&y)[9], int r, double alpha
 
 // kTpw consecutive tiles per workgroup of kTpw x kThreads threads: part
 // w = tid / kThreads of the workgroup takes tile kTpw g + w of tile group g,
-// with the tile kernel's lanes and LDS layout.  kTpw = 4 keeps the partials
-// of the N = 100k grid city (1,697 tiles) at one per group (425 <= 512);
-// held to 64 VGPRs, two such groups share a CU, so all of them run in one
-// round.  Groups map onto the XCDs contiguously (neighbouring rows in one
-// L2), and the partials are indexed by group.
+// with the tile kernel's lanes and LDS layout.  kTpw = 2 and 4 keep the
+// partials at one per group and thread (the N = 100k grid city: 1,697
+// tiles, 425 groups of four); held to 64 VGPRs, four groups of two or two
+// of four share a CU, so all of them run in one round.  Groups map onto the
+// XCDs contiguously (neighbouring rows in one L2; the dispatcher's round
+// robin measured 16.5 against 15.8 us per iteration at N = 100k), and the
+// partials are indexed by group.
 template <int kTpw>
 __global__ void __launch_bounds__(kTpw * kThreads)
 __attribute__((amdgpu_waves_per_eu(kTpw == 1 ? 1 : 8, 8)))
@@ -1001,17 +1003,20 @@ int pose_update(DeviceGraph &g, double *diff_out)
 
 // One kernel per iteration (pcg_fused_kernel) for graphs of at most
 // kFusedMaxTiles tiles: one tile per workgroup up to kFusedMaxGroups tiles,
-// kFusedWideTiles beyond.
+// two up to twice that, four beyond -- the fewest tiles per workgroup that
+// keep the partials at one per thread and quantity (a second partial load
+// per thread cost ~1 us per iteration at N = 40k, more than the larger
+// workgroups' barriers: 8.65 us against 9.42 with four tiles and 9.69 with
+// two tiles and two loads).
 static int pcg_solve_fused(DeviceGraph &g, const gslam_solve_opts &o, const double *b,
                            double *x_out, gslam_solve_stats &s)
 {
     const int32_t n = g.h.n;
     const unsigned int ge = grid_for(n, 1);
-    const bool wide = g.n_tiles > kFusedMaxGroups;
-    const int tpw = wide ? kFusedWideTiles : 1;
+    const int tpw = g.n_tiles <= kFusedMaxGroups ? 1 : (g.n_tiles <= 2 * kFusedMaxGroups ? 2 : 4);
     const unsigned int gt = static_cast<unsigned int>((g.n_tiles + tpw - 1) / tpw);
     const dim3 blk(tpw * kThreads);
-    auto kern = wide ? pcg_fused_kernel<kFusedWideTiles> : pcg_fused_kernel<1>;
+    auto kern = tpw == 1 ? pcg_fused_kernel<1> : (tpw == 2 ? pcg_fused_kernel<2> : pcg_fused_kernel<4>);
     PcgStatus *st = g.status;
     GSLAM_TRY(g.tval_reserve());  // first solve: outside the timed region
     GSLAM_TRY(g.fused_reserve());

@@ -33,10 +33,10 @@ constexpr int kUpdThreads = 192;  // entry-parallel update: 64 poses per pass
 // kFusedParts partials of the previous launch (one per workgroup, at most
 // kFusedMaxGroups; 10 loads per thread at most), so the update of r, z, p
 // and x needs no launch of its own.  A workgroup takes one tile up to
-// kFusedMaxGroups tiles, kFusedWideTiles consecutive tiles beyond.
+// kFusedMaxGroups tiles, two consecutive tiles up to twice that, four
+// beyond (kFusedMaxTiles).
 constexpr int kFusedMaxGroups = 512;
-constexpr int kFusedWideTiles = 4;
-constexpr int kFusedMaxTiles = kFusedWideTiles * kFusedMaxGroups;
+constexpr int kFusedMaxTiles = 4 * kFusedMaxGroups;
 constexpr int kFusedParts = 5;  // p.q, r.z, q.z, q.D^-1 q, r.r
 
 // Arrays that a PCG kernel reads once per iteration (the matrix values and
