@@ -592,12 +592,22 @@ pcg_fused_kernel(const int32_t *__restrict__ brow_ptr, const int32_t *__restrict
     // kernel, np_in <= kFusedMaxGroups), as unconditional loads from clamped
     // indices, summed below in the same order in every workgroup
     double pa[kLoads][kFusedParts];
+    // (four tiles per workgroup: waves past the partials load none -- 9 of 16
+    // at N = 100k, 15.88 -> 15.50 us per iteration; with one or two tiles
+    // the branch costs more than the loads it saves, 5.47 -> 5.54 and 8.66
+    // -> 9.02 us at N = 10k and 40k)
+    const int wave_first = __builtin_amdgcn_readfirstlane(tid);
 #pragma unroll
     for (int u = 0; u < kLoads; ++u) {
         const uint32_t iu = 8u * static_cast<uint32_t>(min(tid + u * NT, np_in - 1));
+        if (kTpw < 4 || wave_first + u * NT < np_in) {
 #pragma unroll
-        for (int k = 0; k < kFusedParts; ++k)
-            pa[u][k] = *at_byte(part_in + (L == 0 ? 0 : k * kFusedMaxGroups), iu);
+            for (int k = 0; k < kFusedParts; ++k)
+                pa[u][k] = *at_byte(part_in + (L == 0 ? 0 : k * kFusedMaxGroups), iu);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kFusedParts; ++k) pa[u][k] = 0.0;
+        }
     }
     asm volatile("" ::: "memory");  // keeps the loads above here (not sunk to their use)
     // this lane's block (tile-padded slot: zeros and a valid column in padding)
