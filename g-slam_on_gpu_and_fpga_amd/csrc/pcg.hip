@@ -435,9 +435,10 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
 }
 
 // One launch L per PCG iteration for graphs of at most kFusedMaxTiles SpMV
-// tiles (one workgroup per tile), where two launches per iteration, not the
-// bytes, bound the solve.  Launch L completes iteration L-1 and starts
-// iteration L of the recurrence above:
+// tiles (one, two or four tiles per workgroup), where the dependent
+// launches of the two-launch form, not the bytes, bound the iteration.
+// Launch L completes iteration L-1 and starts iteration L of the recurrence
+// above:
 //   from the partials of launch L-1 -- p.q, r.z, q.z, q.w and r.r of
 //   p_(L-1), q_(L-1) = A p_(L-1), r_(L-1), z_(L-1) = D^-1 r_(L-1) and
 //   w_(L-1) = D^-1 q_(L-1) --, reduced by every workgroup in the same order:
@@ -446,12 +447,15 @@ pcg_update_kernel(const double *__restrict__ q, const double *__restrict__ dinv,
 //     alpha = r.z / p.q;
 //     r.z_L = r.z - 2 alpha q.z + alpha^2 q.w, which is r_L . D^-1 r_L for
 //       r_L = r_(L-1) - alpha q_(L-1) (D^-1 is symmetric); beta = r.z_L / r.z;
-//   then p_L = (z_(L-1) - alpha w_(L-1)) + beta p_(L-1) for every gathered
-//   column (its record {z, w, p} of launch L-1: one 72-byte run per pose) and
-//   its own rows, the same arithmetic everywhere; for its own rows r_L,
-//   x_L = x_(L-1) + alpha p_(L-1), q_L = A p_L, z_L = D^-1 r_L and
-//   w_L = D^-1 q_L (the pose's three entries through LDS), the record of
-//   launch L and its partials.
+//   then, for every block, V p_L = V z_(L-1) - alpha V w_(L-1) +
+//   beta V p_(L-1) from the record {z, w, p} of launch L-1 of its column
+//   (one 72-byte run per pose; the three products formed as the loads
+//   arrive), summed per row into q_L = A p_L; for its own rows
+//   p_L = (z_(L-1) - alpha w_(L-1)) + beta p_(L-1), r_L,
+//   x_L = x_(L-1) + alpha p_(L-1), z_L = D^-1 r_L and w_L = D^-1 q_L (the
+//   pose's three entries through LDS), the record of launch L and its
+//   partials.  (q_L is A p_L up to the rounding of the products' order, as
+//   in the two-launch SpMV, which forms V z + beta V p_old.)
 // z_(L-1) - alpha w_(L-1) is D^-1 r_L up to the rounding of this one step
 // (z is recomputed from r by every launch, so nothing accumulates); r.z_L is
 // reduced from the recomputed z_L.  Launch 0 starts from r_(-1) = b,
