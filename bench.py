@@ -698,6 +698,17 @@ def profile_kernel(args, gslam, G):
         gbs = 2 * (1 << 31) / (us * 1e-6) / 1e9
         log(f"stream: 2 GiB copy {us:.0f} us, {gbs:.0f} GB/s read+write = "
             f"{100 * gbs / HBM_PEAK_GBS:.1f}% of the 8 TB/s HBM peak")
+        del a, b
+        # reads and copies of working sets the size of the PCG iteration's (69
+        # MB at N = 100k), re-read by every launch as the iteration re-reads
+        # its matrix and vectors: what plain streaming kernels achieve there
+        probe = os.path.join(ROOT, "g-slam_on_gpu_and_fpga_amd", "bin", "gslam_stream_probe")
+        if not os.path.exists(probe):
+            raise SystemExit(f"{probe} is not built (python -c 'import __graft_entry__ as e; e.build()')")
+        torch.cuda.synchronize()
+        out = subprocess.run([probe, "31", "69", "1024"], check=True, capture_output=True, text=True)
+        for line in out.stdout.splitlines():
+            log(line)
         return
     if args.profile_kernel == "spmv_mixed":
         gc = G.grid_city(args.n, seed=args.seed)
