@@ -24,7 +24,8 @@
  * The OpenMP pragmas in the PCG loops take effect only in liboracle_omp.so
  * (the same file built with -fopenmp), a multi-core CPU timing baseline whose
  * dot products are summed in another order: within rounding of liboracle.so
- * (tests/test_oracle_omp.py), not bitwise.  liboracle.so ignores them.
+ * (tests/test_oracle_omp.py), not bitwise, and the same bits on every run
+ * for a given team size.  liboracle.so ignores them.
  */
 #include "gslam_oracle.h"
 
@@ -34,6 +35,33 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+#ifdef _OPENMP
+/* Sum over i in [0, n) of the terms that BODY adds to s, [0, n) split into
+ * one contiguous range per thread and the per-thread sums added in thread
+ * order: the same bits on every run for a given team size (a reduction
+ * clause adds them in the order in which the threads finish, which moved the
+ * converged PCG iteration count on the victoria-park-100 system over
+ * 380-394). */
+#define ORC_MAX_THREADS 256
+#define ORC_OMP_SUM(out, n, BODY)                                             \
+    do {                                                                      \
+        double part_[ORC_MAX_THREADS];                                        \
+        int nt_ = 1;                                                          \
+        int req_ = omp_get_max_threads();                                     \
+        if (req_ > ORC_MAX_THREADS) req_ = ORC_MAX_THREADS;                   \
+        _Pragma("omp parallel num_threads(req_)")                             \
+        {                                                                     \
+            const int t_ = omp_get_thread_num(), T_ = omp_get_num_threads();  \
+            const int64_t lo_ = (n) * t_ / T_, hi_ = (n) * (t_ + 1) / T_;     \
+            double s = 0.0;                                                   \
+            for (int64_t i = lo_; i < hi_; i++) { BODY; }                     \
+            part_[t_] = s;                                                    \
+            if (t_ == 0) nt_ = T_;                                            \
+        }                                                                     \
+        for (int t_ = 0; t_ < nt_; t_++) (out) += part_[t_];                  \
+    } while (0)
 #endif
 
 /* Threads the PCG loops use: 1 in liboracle.so, the OpenMP team size in
@@ -710,24 +738,33 @@ int32_t orc_pcg(int32_t dim, const int32_t *row_ptr, const int32_t *col_idx,
     for (int32_t k = 0; k < max_iter; k++) {                        /* :347 */
         orc_spmv(dim, row_ptr, col_idx, values, p, q);
         double pq = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : pq)
+#ifdef _OPENMP
+        ORC_OMP_SUM(pq, n, s += p[i] * q[i]);
+#else
         for (int64_t i = 0; i < n; i++) pq += p[i] * q[i];
+#endif
         if (pq == 0.0) break;                                       /* :357 */
         double alpha = rz_old / pq;
         double r_norm = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : r_norm)
+#ifdef _OPENMP
+        ORC_OMP_SUM(r_norm, n, x[i] += alpha * p[i]; r[i] -= alpha * q[i]; s += r[i] * r[i]);
+#else
         for (int64_t i = 0; i < n; i++) {
             x[i] += alpha * p[i];
             r[i] -= alpha * q[i];
             r_norm += r[i] * r[i];
         }
+#endif
         it = k + 1;
         if (res_hist) res_hist[k] = sqrt(r_norm);
         if (sqrt(r_norm) < tol) break;                              /* :366 */
         apply_bj(n, diag_inv, r, z);
         double rz_new = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : rz_new)
+#ifdef _OPENMP
+        ORC_OMP_SUM(rz_new, n, s += r[i] * z[i]);
+#else
         for (int64_t i = 0; i < n; i++) rz_new += r[i] * z[i];
+#endif
         double beta = rz_new / rz_old;
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];

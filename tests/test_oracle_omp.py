@@ -10,7 +10,10 @@ thread and then across threads.  Checks:
   iterations at N=20k) -- a race on a reduction or a vector gives O(1);
 * converged to 1e-10 on the reference's victoria-park-100 system, the same
   solution within 1e-8 and an iteration count within 2% (rounding changes
-  the CG trajectory: 391 vs 387 iterations, 2.9e-11 apart).
+  the CG trajectory: 391 vs 389 iterations with 8 threads, 2.4e-11 apart);
+* the OpenMP sums are deterministic: two runs give the same bits (the
+  per-thread sums are added in thread order; a reduction clause adds them as
+  the threads finish, and moved that count over 380-394).
 """
 import numpy as np
 import pytest
@@ -53,3 +56,11 @@ def test_converged_reference_system(oracle, omp):
     xp, itp = omp.pcg(*args)
     assert abs(itp - its) <= 0.02 * its, (its, itp)
     assert rel(xp, xs) <= 1e-8, rel(xp, xs)
+
+
+def test_runs_repeat_bitwise(omp):
+    fx = golden("vp100")
+    args = (fx["row_ptr"], fx["col_idx"], fx["values"], fx["xi"], 1e-10, 5000)
+    runs = [omp.pcg(*args) for _ in range(20)]
+    assert len({it for _, it in runs}) == 1
+    assert all(np.array_equal(x, runs[0][0]) for x, _ in runs)
